@@ -1,0 +1,273 @@
+"""Benchmark: per-pixel SVBRDF optimisation iterations/sec at 1024^2 x 9 views.
+
+One step = one fused iteration of the reference loop body (src/svbrdf.py:58-71):
+clamp -> render -> log-space L2 loss -> backward -> Adam, on a 1024x1024
+material seen from 9 co-located flash views, all fp32, as one kernel launch
+replayed from a HIP graph.  Inputs are synthetic (seeded maps, targets
+rendered from the ground truth) and resident in HBM before timing starts.
+
+Printed JSON line (one):
+  value             iterations/sec of the fused step (graph replays, HIP events)
+  roofline          algorithmic HBM bytes per iteration (324 B/pixel at V=9:
+                    params, exp_avg, exp_avg_sq read+written once, targets read
+                    once) / measured kernel time, against 8 TB/s
+  torch_gpu_baseline  the reference's eager op sequence + torch.optim.Adam on
+                    the same GPU (it/s), and the speedup over it
+  cpu_baseline      the same eager restatement on the host CPU cores (oracle/)
+  forward           forward-render throughput at the same shape (config 1)
+
+    python bench.py [--steps K] [--warmup W] [--res 1024] [--views 9]
+                    [--no-cpu] [--no-torch] [--cpu-seconds 20]
+"""
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import __graft_entry__  # noqa: E402
+
+METRIC = "per-pixel optim iters/sec at 1024²×9 views (fused vs torch autograd); HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_step(res, views):
+    # params, exp_avg, exp_avg_sq: 9 fp32 each, read and written once (216 B)
+    # targets: 3 fp32 per view read once (12 V B)
+    return res * res * (216 + 12 * views)
+
+
+def algorithmic_bytes_render(res, views):
+    return res * res * (36 + 12 * views)
+
+
+def make_problem(pkg, res, views, dev, loss="log_l2"):
+    syn = pkg.synthetic
+    gt = syn.ground_truth_textures(res, seed=0)
+    init = syn.init_textures(gt, seed=1)
+    cam, light = syn.flash_grid(n_views=views, jitter=3.0 if views != 9 else 0.0,
+                                rng=np.random.default_rng(3))
+    P = syn.light_power()
+    cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+    r = pkg.Microfacet(res, views, syn.PLANE_SIZE_CM, cl, dev)
+    with torch.no_grad():
+        target = r.eval(torch.tensor(gt, device=dev)[None])
+    tex = torch.tensor(init, device=dev)[None].contiguous()
+    fs = pkg.FusedStep(r, tex, target, lr=0.01, loss=loss, history=0)
+    return dict(renderer=r, fused=fs, tex=tex, target=target, gt=gt, init=init, cam=cam, light=light, P=P)
+
+
+def time_events(fn, n, stream=None):
+    """Average ms per call of fn() over n calls, with HIP events on the stream
+    the kernels are launched on (torch's current stream)."""
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(n):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / n
+
+
+# --------------------------------------------------------------------------
+# The reference's eager op sequence on the GPU (torch autograd + Adam): the
+# denominator of the speedup.  Same ops as src/microfacet.py:84-120, one
+# torch op per term on expanded [V,3,R,R] tensors.
+class EagerReferencePath:
+    def __init__(self, res, n, size, cam, light, P, dev):
+        t = torch.arange(res, dtype=torch.float32, device=dev)
+        t = ((t + 0.5) / res - 0.5) * size
+        x, y = torch.meshgrid(t, t, indexing="xy")
+        self.pos = torch.stack((x, -y, torch.zeros_like(x)), 2).permute(2, 0, 1)[None].expand(n, -1, -1, -1)
+        self.cam = torch.tensor(cam, device=dev)[:, :, None, None].expand(-1, -1, res, res)
+        self.light = torch.tensor(light, device=dev)[:, :, None, None].expand(-1, -1, res, res)
+        self.P = torch.tensor(P, device=dev)[None, :, None, None].expand(n, -1, res, res)
+        self.n = n
+
+    @staticmethod
+    def dot(a, b):
+        return (a * b).sum(1, keepdim=True).expand(-1, 3, -1, -1)
+
+    @staticmethod
+    def normalize(v):
+        return v / v.norm(2.0, 1, keepdim=True)
+
+    def eval(self, tex):
+        n = self.n
+        dif = (((tex[:, 0:3] + 1) / 2) ** 2.2).expand(n, -1, -1, -1)
+        nx, ny = tex[:, 3].clamp(-1, 1), tex[:, 4].clamp(-1, 1)
+        nz = (1 - (nx ** 2 + ny ** 2).clamp(0, 1 - 1e-6)).sqrt()
+        nrm = self.normalize(torch.stack((nx, ny, nz), 1)).expand(n, -1, -1, -1)
+        rgh = (((tex[:, 5] + 1) / 2) ** 2.2).expand(n, 3, -1, -1)
+        spe = (((tex[:, 6:9] + 1) / 2) ** 2.2).expand(n, -1, -1, -1)
+        vv = self.cam - self.pos
+        v = self.normalize(vv)
+        lv = self.light - self.pos
+        l, d2 = self.normalize(lv), self.dot(lv, lv)
+        h = self.normalize(l + v)
+        nv = self.dot(nrm, v).clamp(min=0)
+        nl = self.dot(nrm, l).clamp(min=0)
+        nh = self.dot(nrm, h).clamp(min=0)
+        vh = self.dot(v, h).clamp(min=0)
+        f1 = dif / math.pi * (1 - spe)
+        alpha = rgh ** 2
+        c2 = nh ** 2
+        a2 = alpha ** 2
+        D = a2 / (math.pi * (c2 * a2 + (1 - c2)) ** 2 + 1e-6)
+        F = spe + (1.0 - spe) * torch.pow(2.0, ((-5.55473 * vh) - 6.98316) * vh)
+        k = rgh ** 2 * 0.5 + 1e-6
+        G = nv / (nv * (1.0 - k) + k) * (nl / (nl * (1.0 - k) + k))
+        f = f1 + D * F * G / (4 * nv * nl + 1e-6)
+        img = self.P * f * nl / d2
+        return img.clamp(1e-6, 1) ** (1 / 2.2)
+
+
+def torch_gpu_baseline(prob, res, views, dev, steps=5, warmup=2, sync=True):
+    path = EagerReferencePath(res, views, 17.0, prob["cam"], prob["light"], prob["P"], dev)
+    tex = torch.tensor(prob["init"], device=dev)[None].requires_grad_(True)
+    log_tgt = torch.log(prob["target"].detach().clamp(1e-6 ** (1 / 2.2), 1))
+    opt = torch.optim.Adam([tex], lr=0.01, betas=(0.9, 0.999))
+
+    def it():
+        out = path.eval(tex.clamp(-1, 1))
+        loss = torch.nn.functional.mse_loss(torch.log(out), log_tgt)
+        if sync:
+            loss.item()                   # the reference reads the loss every epoch (src/svbrdf.py:64)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+
+    for _ in range(warmup):
+        it()
+    ms = time_events(it, steps)
+    return {"value": 1000.0 / ms, "unit": "it/s", "ms_per_step": ms, "steps": steps,
+            "what": "reference op sequence (src/microfacet.py:84-120) + log-L2 + torch.optim.Adam, eager, "
+                    "autograd, per-iteration loss.item() as src/svbrdf.py:64"}
+
+
+def cpu_baseline(res, views, seconds):
+    """The oracle's eager restatement on the host cores: the reference's CPU
+    path (src/scripts.py:68 falls back to device 'cpu')."""
+    from oracle import microfacet_torch as MT   # the CPU baseline leg only
+    pkg = __graft_entry__.load_package()
+    syn = pkg.synthetic
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    torch.set_num_threads(threads)
+    gt = syn.ground_truth_textures(res, seed=0)
+    init = syn.init_textures(gt, seed=1)
+    cam, light = syn.flash_grid(n_views=views)
+    P = syn.light_power()
+    cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+    r = MT.MicrofacetEager(res, views, syn.PLANE_SIZE_CM, cl, "cpu")
+    with torch.no_grad():
+        target = r.eval(torch.tensor(gt)[None])
+    tex = torch.tensor(init)[None].requires_grad_(True)
+    opt = torch.optim.Adam([tex], lr=0.01, betas=(0.9, 0.999))
+    log_tgt = torch.log(target.clamp(MT.target_floor(), 1))
+    times = []
+    t_end = time.perf_counter() + seconds
+    while True:
+        t0 = time.perf_counter()
+        out = r.eval(tex.clamp(-1, 1))
+        loss = torch.nn.functional.mse_loss(torch.log(out), log_tgt)
+        loss.item()
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 50:
+            break
+    timed = times[1:] if len(times) > 1 else times
+    s = float(np.median(timed))
+    return {"value": 1.0 / s, "unit": "it/s", "cores": threads, "kind": "port",
+            "sample": f"{len(timed)} timed iteration(s) (+1 warm-up) of the eager restatement at {res}^2 x "
+                      f"{views} views, log-L2 + torch.optim.Adam, median s/it = {s:.3f}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--res", type=int, default=1024)
+    ap.add_argument("--views", type=int, default=9)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-torch", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pkg = __graft_entry__.load_package()
+    dev = torch.device("cuda:0")
+    res, views = args.res, args.views
+    prob = make_problem(pkg, res, views, dev)
+    fs = prob["fused"]
+
+    # fused step: capture one iteration, warm up, time K replays
+    if args.eager:
+        run = fs.step
+    else:
+        fs.capture()
+        run = fs.replay
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    ms = time_events(run, args.steps)
+    loss_after = fs.loss()
+    value = 1000.0 / ms
+
+    nbytes = algorithmic_bytes_step(res, views)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "it/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"fused per-pixel iteration (clamp+render+log-L2+adjoint+Adam) {res}^2 x {views} "
+                               f"views, one kernel per step, HIP-graph replay",
+                   "res": res, "views": views, "loss": "log_l2", "graph": not args.eager},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": nbytes, "kernel": "fused_step_kernel"},
+        "loss_after": loss_after,
+    }
+
+    # forward render at the same shape (config 1)
+    r = prob["renderer"]
+    gt = torch.tensor(prob["gt"], device=dev)[None]
+    with torch.no_grad():
+        for _ in range(5):
+            r.eval(gt)
+        fms = time_events(lambda: r.eval(gt), 50)
+    fb = algorithmic_bytes_render(res, views)
+    out["forward"] = {"value": round(1000.0 / fms, 1), "unit": "renders/s", "ms": round(fms, 5),
+                      "achieved_GBs": round(fb / (fms * 1e-3) / 1e9, 1),
+                      "frac": round(fb / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    if not args.no_torch:
+        try:
+            tb = torch_gpu_baseline(prob, res, views, dev)
+            out["torch_gpu_baseline"] = tb
+            out["speedup_vs_torch_gpu"] = round(value / tb["value"], 1)
+        except torch.cuda.OutOfMemoryError as e:     # pragma: no cover
+            out["torch_gpu_baseline"] = {"error": str(e)[:200]}
+    if not args.no_cpu:
+        cb = cpu_baseline(res, views, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

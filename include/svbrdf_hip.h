@@ -1,0 +1,133 @@
+/*
+ * svbrdf_hip.h -- C ABI of the MI355X (gfx950) microfacet renderer and
+ * per-pixel SVBRDF optimisation step.
+ *
+ * The reference has no FFI: its boundary is the duck-typed Python renderer
+ * object held by the optimiser (src/optimization.py:15).  Each entry point
+ * below replaces one operation of that object or of the optimiser loop that
+ * drives it; the Python wrapper in svbrdf-diff-renderer_amd/ binds them with
+ * ctypes and keeps the reference's class surface.
+ *
+ * Conventions
+ *   - All tensor pointers are DEVICE pointers to contiguous fp32 arrays in
+ *     the reference's layouts: textures [9,R,R] (channels: diffuse 0:3,
+ *     normal xy 3:5, roughness 5, specular 6:9; src/svbrdf.py:166),
+ *     images [V,3,R,R], camera_pos / light_pos [V,3] in cm, light_pow [3].
+ *   - The library borrows every pointer for the duration of a stream-ordered
+ *     call and allocates nothing.  Work that needs scratch memory takes a
+ *     caller-owned workspace whose size the *_workspace_bytes() query gives.
+ *   - Every call is asynchronous on `stream` (a hipStream_t passed as void*;
+ *     NULL = the default stream) and contains no host synchronisation, so it
+ *     can be captured into a HIP graph.
+ *   - Return value: SVBRDF_OK or a negative status; svbrdf_last_error()
+ *     returns a message for the calling thread's last failure.  Arguments are
+ *     validated on the host before anything is launched.
+ */
+#ifndef SVBRDF_HIP_H
+#define SVBRDF_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVBRDF_ABI_VERSION 1
+
+enum svbrdf_status {
+    SVBRDF_OK = 0,
+    SVBRDF_EINVAL = -1,   /* null pointer, bad size or bad enum value         */
+    SVBRDF_ESHAPE = -2,   /* resolution / view count out of range             */
+    SVBRDF_ESPACE = -3,   /* workspace too small                              */
+    SVBRDF_EHIP = -4,     /* a HIP runtime call failed (message has details)  */
+};
+
+enum svbrdf_loss {
+    SVBRDF_LOSS_L2 = 0,      /* mean((out - target)^2): th.nn.MSELoss, src/optimization.py:14,29 */
+    SVBRDF_LOSS_LOG_L2 = 1,  /* mean((ln out - ln clamp(target, eps^(1/2.2), 1))^2)             */
+};
+
+/* Device-resident optimiser state of the fused step.  Lives in device
+ * memory (svbrdf_step_state_bytes() bytes, 16-byte aligned), is set up by
+ * svbrdf_step_state_init() and advanced by every svbrdf_fused_step() call,
+ * so a captured graph replays correctly without host-side scalars. */
+typedef struct svbrdf_step_state {
+    double bias_correction1;     /* 1 - beta1^t for the step the next call performs */
+    double bias_correction2;     /* 1 - beta2^t                                     */
+    float step;                  /* completed steps (torch keeps it as a float)     */
+    float loss;                  /* image loss evaluated by the last completed step */
+    unsigned int blocks_done;    /* internal: blocks finished in the running call   */
+    unsigned int n_partials;     /* internal: partial sums written by the last call */
+    float light_exp_avg[4];      /* Adam state of light_pow (when optimised)        */
+    float light_exp_avg_sq[4];
+    float light_grad[4];         /* dL/dlight_pow of the last step (when optimised) */
+} svbrdf_step_state;
+
+int svbrdf_abi_version(void);
+const char *svbrdf_last_error(void);
+size_t svbrdf_step_state_bytes(void);
+
+/* Largest resolution and view count accepted. */
+int svbrdf_max_res(void);
+int svbrdf_max_views(void);
+
+/* Forward render.  Replaces Microfacet.eval(textures) (src/microfacet.py:84-120)
+ * with the light power set by Microfacet.update_light (src/microfacet.py:81-82)
+ * and the geometry of Microfacet.__init__ (src/microfacet.py:10-24).
+ * images[v,c,i,j] = clamp(P_c f_c(n.l)/|l|^2, 1e-6, 1)^(1/2.2). */
+int svbrdf_render(const float *textures, int res, int n_views, float size,
+                  const float *camera_pos, const float *light_pos,
+                  const float *light_pow, float *images, void *stream);
+
+/* Adjoint of svbrdf_render: the backward of Microfacet.eval under torch
+ * autograd (src/microfacet.py:64-120).  Given dL/dimages it writes
+ * dL/dtextures (overwritten, not accumulated) and, when grad_light_pow is not
+ * NULL, dL/dlight_pow[3] (needs a workspace of
+ * svbrdf_render_backward_workspace_bytes(res) bytes; otherwise workspace may
+ * be NULL).  The BRDF terms are recomputed, not stored. */
+size_t svbrdf_render_backward_workspace_bytes(int res);
+int svbrdf_render_backward(const float *textures, int res, int n_views, float size,
+                           const float *camera_pos, const float *light_pos,
+                           const float *light_pow, const float *grad_images,
+                           float *grad_textures, float *grad_light_pow,
+                           void *workspace, size_t workspace_bytes, void *stream);
+
+/* Target preparation for the fused step (once per target set).  For
+ * SVBRDF_LOSS_LOG_L2 writes ln(clamp(targets, 1e-6^(1/2.2), 1)); for
+ * SVBRDF_LOSS_L2 copies.  `prepared` may alias `targets`. */
+int svbrdf_prepare_targets(const float *targets, size_t count, int loss_mode,
+                           float *prepared, void *stream);
+
+/* Resets a svbrdf_step_state to "no step taken" for the given betas (as
+ * torch.optim.Adam's lazily created state, torch/optim/adam.py:142-164). */
+int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream);
+
+/* One fused per-pixel optimisation iteration: the body of the loop of
+ * SvbrdfOptim.optim (src/svbrdf.py:58-71):
+ *   clamp(textures,-1,1) -> render -> image loss -> backward -> Adam step,
+ * with Adam = torch.optim.Adam(lr, betas, eps), no weight decay, no amsgrad.
+ * textures / exp_avg / exp_avg_sq ([9,R,R]) are updated in place; the stored
+ * textures stay unclamped (the clamp is applied on read, with torch's
+ * inclusive-bound gradient mask).  targets_prepared comes from
+ * svbrdf_prepare_targets with the same loss_mode.  When optim_light is
+ * non-zero light_pow[3] is optimised too (src/svbrdf.py:48-50,58-59) with
+ * its Adam state in `state`.  The loss of this step is left in state->loss
+ * and, when loss_history is not NULL and the step index t (0-based) is below
+ * history_len, also in loss_history[t] (device memory), so a loop can keep
+ * every loss without a host synchronisation per iteration (the reference
+ * reads loss.item() each epoch, src/svbrdf.py:64).  The workspace holds
+ * per-block partial sums (svbrdf_fused_step_workspace_bytes(res) bytes). */
+size_t svbrdf_fused_step_workspace_bytes(int res);
+int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq,
+                      const float *targets_prepared, int res, int n_views, float size,
+                      const float *camera_pos, const float *light_pos, float *light_pow,
+                      int loss_mode, int optim_light, double lr, double beta1,
+                      double beta2, double eps, void *state, float *loss_history,
+                      int history_len, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVBRDF_HIP_H */

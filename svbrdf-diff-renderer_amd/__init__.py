@@ -1,0 +1,20 @@
+"""MI355X-native microfacet renderer and per-pixel SVBRDF optimiser.
+
+Drop-in for the hot path of tflsguoyu/svbrdf-diff-renderer:
+
+* :class:`Microfacet` -- src/microfacet.py:9-120 (forward + adjoint kernels)
+* :class:`SvbrdfOptim` -- src/svbrdf.py:15-83 (one fused kernel per iteration)
+* :class:`FusedStep` -- the device-resident optimisation problem behind it
+  (graph capture, exact resume)
+
+The directory name is not a Python identifier; it is imported under the
+module name ``svbrdf_amd`` (see ``load_package`` in __graft_entry__.py).
+All compute runs in libsvbrdf_hip.so (gfx950); nothing falls back to the CPU.
+"""
+from . import _lib, synthetic
+from ._lib import LOSS_L2, LOSS_LOG_L2, SvbrdfHipError
+from .optim import FusedStep, SvbrdfOptim, loss_mode
+from .renderer import Microfacet, RenderFunction
+
+__all__ = ["Microfacet", "RenderFunction", "SvbrdfOptim", "FusedStep", "loss_mode", "synthetic",
+           "LOSS_L2", "LOSS_LOG_L2", "SvbrdfHipError"]

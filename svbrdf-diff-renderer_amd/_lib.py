@@ -1,0 +1,96 @@
+"""ctypes binding of libsvbrdf_hip.so (the C ABI in include/svbrdf_hip.h).
+
+The library is the only compute path: if it is missing or a call fails, the
+wrapper raises.  There is no CPU or eager-torch fallback.
+"""
+import ctypes
+import os
+import pathlib
+
+import torch
+
+_PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsvbrdf_hip.so"
+
+OK = 0
+EINVAL = -1
+ESHAPE = -2
+ESPACE = -3
+EHIP = -4
+
+LOSS_L2 = 0
+LOSS_LOG_L2 = 1
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes): every symbol declared in include/svbrdf_hip.h
+SIGNATURES = {
+    "svbrdf_abi_version": (ctypes.c_int, []),
+    "svbrdf_last_error": (ctypes.c_char_p, []),
+    "svbrdf_step_state_bytes": (ctypes.c_size_t, []),
+    "svbrdf_max_res": (ctypes.c_int, []),
+    "svbrdf_max_views": (ctypes.c_int, []),
+    "svbrdf_render": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp, _vp, _vp, _vp]),
+    "svbrdf_render_backward_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),
+    "svbrdf_render_backward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp, _vp,
+                                              _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "svbrdf_prepare_targets": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
+    "svbrdf_step_state_init": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp]),
+    "svbrdf_fused_step_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),
+    "svbrdf_fused_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp,
+                                         _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp,
+                                         ctypes.c_size_t, _vp]),
+}
+
+_lib = None
+
+
+class SvbrdfHipError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Loads the shared library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python __graft_entry__.py build` "
+                          "(or svbrdf-diff-renderer_amd/csrc/build.py)")
+    lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Calls an int-returning entry point and raises on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != OK:
+        msg = lib.svbrdf_last_error().decode(errors="replace")
+        if rc in (EINVAL, ESHAPE, ESPACE):
+            raise ValueError(f"{name}: {msg}")
+        raise SvbrdfHipError(name, rc, msg)
+
+
+def query(name, *args):
+    return getattr(load(), name)(*args)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

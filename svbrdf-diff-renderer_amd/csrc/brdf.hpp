@@ -1,0 +1,272 @@
+// brdf.hpp -- per-pixel microfacet shading and its adjoint, gfx950 device code.
+//
+// One thread owns one pixel: it decodes the 9 texture channels once, walks
+// the views, and keeps every per-view sum (gradients over views and RGB) in
+// registers, so no [V,3,R,R] intermediate ever reaches memory.
+//
+// Math follows src/microfacet.py:
+//   decode      :64-79   u = (t+1)/2, map = u^2.2; normal from (nx, ny)
+//   geometry    :16-24, :57-62, :91-93 (v, l, |l|^2, h)
+//   dots        :54-55, :96-99 (clamped at 0)
+//   GGX         :28-32   D = a2 / (pi den^2 + eps), a2 = rgh^4
+//   Fresnel     :43-45   F = spe + (1-spe) 2^((-5.55473 c - 6.98316) c)
+//   Smith       :47-52   G = G1(n.v) G1(n.l), k = rgh^2/2 + eps
+//   radiance    :101-117 img = P (dif/pi (1-spe) + D F G/(4 nv nl + eps)) nl / d2
+//   output      :120     clamp(img, eps, 1)^(1/2.2)
+// and the backward follows torch autograd's conventions for that op
+// sequence (inclusive clamp masks, sqrt' = 1/(2 sqrt), full Jacobian of the
+// normal normalisation, expand-backward sums over views and channels).
+//
+// Transcendentals use the hardware instructions (v_rcp/v_rsq/v_sqrt/v_exp/
+// v_log, ~1 ulp); every input that reaches them is either 0 or a normal
+// number (u >= 2^-25 or 0, img >= 1e-6), so no denormal handling is needed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace svbrdf {
+
+constexpr float kEps = 1e-6f;                       // microfacet.py:14
+constexpr float kOneMinusEps = 0.999999f;           // 1 - eps, microfacet.py:67
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kGamma = 2.2f;                      // microfacet.py:74-77
+constexpr float kGammaM1 = 1.2f;
+constexpr float kInvGamma = 0.454545454545454545f;  // 1/2.2, microfacet.py:120
+constexpr float kLn2 = 0.693147180559945309f;
+constexpr float kFresA = -5.55473f;                 // microfacet.py:44
+constexpr float kFresB = -6.98316f;
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+// torch.clamp semantics: NaN passes through (fminf/fmaxf would drop it).
+__device__ __forceinline__ float tclamp(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+__device__ __forceinline__ float tmax0(float x) { return x < 0.0f ? 0.0f : x; }
+
+// Pixel-centre position, microfacet.py:16-19: ((j + 0.5) / R - 0.5) * size,
+// with an IEEE division so non-power-of-two R rounds as the reference does.
+__device__ __forceinline__ float pixel_coord(int j, int res, float size) {
+    return (((float)j + 0.5f) / (float)res - 0.5f) * size;
+}
+
+// Decoded maps of one pixel (values the per-view loop reads).
+struct Maps {
+    float dif[3], spe[3], omspe[3], f1[3];
+    float rgh, alpha, a2, k, omk;
+    float n[3];
+};
+
+// u^2.2 of u = (t+1)/2; u in {0} or >= 2^-25, so log2 is finite or -inf.
+__device__ __forceinline__ float map_pow(float t) {
+    const float u = (t + 1.0f) * 0.5f;
+    return fexp2(kGamma * flog2(u));
+}
+
+// d(u^2.2)/dt = 2.2 u^1.2 / 2 (torch pow_backward: exponent * x^(exponent-1)).
+__device__ __forceinline__ float map_pow_grad(float t) {
+    const float u = (t + 1.0f) * 0.5f;
+    return 0.5f * kGamma * fexp2(kGammaM1 * flog2(u));
+}
+
+__device__ __forceinline__ void decode(const float t[9], Maps &m) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        m.dif[c] = map_pow(t[c]);
+        m.spe[c] = map_pow(t[6 + c]);
+        m.omspe[c] = 1.0f - m.spe[c];
+        m.f1[c] = m.dif[c] * kInvPi * m.omspe[c];
+    }
+    m.rgh = map_pow(t[5]);
+    m.alpha = m.rgh * m.rgh;
+    m.a2 = m.alpha * m.alpha;
+    m.k = m.alpha * 0.5f + kEps;
+    m.omk = 1.0f - m.k;
+    // reconstruct_normal, microfacet.py:64-70
+    const float nx = tclamp(t[3], -1.0f, 1.0f);
+    const float ny = tclamp(t[4], -1.0f, 1.0f);
+    const float s = tclamp(nx * nx + ny * ny, 0.0f, kOneMinusEps);
+    const float nz = fsqrt(1.0f - s);
+    const float il = frsq(nx * nx + ny * ny + nz * nz);
+    m.n[0] = nx * il;
+    m.n[1] = ny * il;
+    m.n[2] = nz * il;
+}
+
+// Per-view geometry of one pixel: v, l, h, 1/|l|^2 and clamp(v.h, 0).
+struct Geom {
+    float v[3], l[3], h[3];
+    float inv_d2, vh;
+    bool coloc;
+};
+
+// `coloc` (camera == light for this view, a wave-uniform fact) makes l = v and
+// h = v/|v| exactly as the general formulas would compute them, with fewer ops.
+__device__ __forceinline__ void geometry(float px, float py, float cx, float cy, float cz,
+                                         float lx, float ly, float lz, bool coloc, Geom &g) {
+    const float vx = cx - px, vy = cy - py, vz = cz;
+    const float sv = vx * vx + vy * vy + vz * vz;
+    const float rv = frsq(sv);
+    g.v[0] = vx * rv; g.v[1] = vy * rv; g.v[2] = vz * rv;
+    g.coloc = coloc;
+    if (coloc) {
+        g.l[0] = g.v[0]; g.l[1] = g.v[1]; g.l[2] = g.v[2];
+        g.inv_d2 = frcp(sv);
+        const float w = g.v[0] * g.v[0] + g.v[1] * g.v[1] + g.v[2] * g.v[2];
+        const float rh = frsq(w);
+        g.h[0] = g.v[0] * rh; g.h[1] = g.v[1] * rh; g.h[2] = g.v[2] * rh;
+        g.vh = tmax0(w * rh);
+    } else {
+        const float ax = lx - px, ay = ly - py, az = lz;
+        const float sl = ax * ax + ay * ay + az * az;
+        const float rl = frsq(sl);
+        g.l[0] = ax * rl; g.l[1] = ay * rl; g.l[2] = az * rl;
+        g.inv_d2 = frcp(sl);
+        const float hx = g.l[0] + g.v[0], hy = g.l[1] + g.v[1], hz = g.l[2] + g.v[2];
+        const float rh = frsq(hx * hx + hy * hy + hz * hz);
+        g.h[0] = hx * rh; g.h[1] = hy * rh; g.h[2] = hz * rh;
+        g.vh = tmax0(g.v[0] * g.h[0] + g.v[1] * g.h[1] + g.v[2] * g.h[2]);
+    }
+}
+
+// Forward BRDF terms of one (pixel, view).
+struct Shade {
+    float nv_raw, nl_raw, nh_raw, nv, nl, nh;
+    float c2, den, iqd, D;
+    float omsphg, F[3];
+    float idv, idl, g1v, g1l, G, iq, DGq;
+    float nl_d2, f[3], img[3];
+};
+
+__device__ __forceinline__ void shade(const Maps &m, const Geom &g, const float P[3], Shade &s) {
+    s.nv_raw = m.n[0] * g.v[0] + m.n[1] * g.v[1] + m.n[2] * g.v[2];
+    if (g.coloc) {
+        s.nl_raw = s.nv_raw;
+    } else {
+        s.nl_raw = m.n[0] * g.l[0] + m.n[1] * g.l[1] + m.n[2] * g.l[2];
+    }
+    s.nh_raw = m.n[0] * g.h[0] + m.n[1] * g.h[1] + m.n[2] * g.h[2];
+    s.nv = tmax0(s.nv_raw);
+    s.nl = tmax0(s.nl_raw);
+    s.nh = tmax0(s.nh_raw);
+    // GGX(n.h, rgh^2)
+    s.c2 = s.nh * s.nh;
+    s.den = s.c2 * m.a2 + (1.0f - s.c2);
+    s.iqd = frcp(kPi * (s.den * s.den) + kEps);
+    s.D = m.a2 * s.iqd;
+    // Fresnel(v.h, spe)
+    const float sphg = fexp2((kFresA * g.vh + kFresB) * g.vh);
+    s.omsphg = 1.0f - sphg;
+    // Smith(n.v, n.l, rgh^2)
+    s.idv = frcp(s.nv * m.omk + m.k);
+    s.g1v = s.nv * s.idv;
+    if (g.coloc) {
+        s.idl = s.idv;
+        s.g1l = s.g1v;
+    } else {
+        s.idl = frcp(s.nl * m.omk + m.k);
+        s.g1l = s.nl * s.idl;
+    }
+    s.G = s.g1v * s.g1l;
+    s.iq = frcp(4.0f * s.nv * s.nl + kEps);
+    s.DGq = s.D * s.G * s.iq;
+    s.nl_d2 = s.nl * g.inv_d2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        s.F[c] = m.spe[c] + m.omspe[c] * sphg;
+        s.f[c] = m.f1[c] + s.DGq * s.F[c];
+        s.img[c] = P[c] * s.f[c] * s.nl_d2;
+    }
+}
+
+// Per-pixel gradient accumulators (sums over views).
+struct Accum {
+    float S[3];    // sum_v dL/df_c            -> diffuse and the f1 part of specular
+    float T[3];    // sum_v dL/dF_c (1 - sphg) -> Fresnel part of specular
+    float A;       // sum_v dL/dalpha          -> roughness
+    float gn[3];   // sum_v dL/dn              -> normal
+    float gP[3];   // sum_v dL/dP_c            -> light power
+};
+
+__device__ __forceinline__ void accum_zero(Accum &a) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { a.S[c] = 0.f; a.T[c] = 0.f; a.gn[c] = 0.f; a.gP[c] = 0.f; }
+    a.A = 0.f;
+}
+
+// Backward of shade() for one view given dL/dimg_c (already masked by the
+// output clamp).  Accumulates into `a`.
+template <bool LIGHT_GRAD>
+__device__ __forceinline__ void shade_backward(const Maps &m, const Geom &g, const Shade &s,
+                                               const float P[3], const float gimg[3], Accum &a) {
+    float W = 0.f, gnl = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float gf = gimg[c] * P[c] * s.nl_d2;
+        a.S[c] += gf;
+        a.T[c] += gf * s.DGq * s.omsphg;
+        W += gf * s.F[c];
+        gnl += gimg[c] * P[c] * s.f[c];
+        if (LIGHT_GRAD) a.gP[c] += gimg[c] * s.f[c] * s.nl_d2;
+    }
+    gnl *= g.inv_d2;
+    // f2 = D F G / q, q = 4 nv nl + eps
+    const float gD = W * s.G * s.iq;
+    const float gG = W * s.D * s.iq;
+    const float gq = -W * s.DGq * s.iq;
+    // G = G1(nv) G1(nl), G1(c) = c / (c (1-k) + k)
+    const float dg1v_dc = m.k * s.idv * s.idv;
+    const float dg1l_dc = m.k * s.idl * s.idl;
+    float gnv = gq * 4.0f * s.nl + gG * s.g1l * dg1v_dc;
+    gnl += gq * 4.0f * s.nv + gG * s.g1v * dg1l_dc;
+    const float gk = -gG * (s.g1l * s.nv * (1.0f - s.nv) * s.idv * s.idv +
+                            s.g1v * s.nl * (1.0f - s.nl) * s.idl * s.idl);
+    // D = a2 / (pi den^2 + eps), den = c2 a2 + 1 - c2
+    const float dD_dden = -2.0f * kPi * m.a2 * s.den * s.iqd * s.iqd;
+    const float ga2 = gD * (s.iqd + dD_dden * s.c2);
+    const float gc2 = gD * dD_dden * (m.a2 - 1.0f);
+    float gnh = gc2 * 2.0f * s.nh;
+    // alpha = rgh^2 feeds k = alpha/2 + eps and a2 = alpha^2
+    a.A += gk * 0.5f + ga2 * 2.0f * m.alpha;
+    // clamp_min(0) masks (inclusive)
+    gnv = s.nv_raw >= 0.0f ? gnv : 0.0f;
+    gnl = s.nl_raw >= 0.0f ? gnl : 0.0f;
+    gnh = s.nh_raw >= 0.0f ? gnh : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.gn[c] += gnv * g.v[c] + gnl * g.l[c] + gnh * g.h[c];
+}
+
+// Texture gradient of one pixel from the view sums: the backward of decode().
+// `t` are the values the forward decoded (already clamped for the fused step).
+__device__ __forceinline__ void decode_backward(const float t[9], const Maps &m, const Accum &a,
+                                                float gt[9]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float gdif = a.S[c] * m.omspe[c] * kInvPi;
+        const float gspe = -a.S[c] * m.dif[c] * kInvPi + a.T[c];
+        gt[c] = gdif * map_pow_grad(t[c]);
+        gt[6 + c] = gspe * map_pow_grad(t[6 + c]);
+    }
+    gt[5] = a.A * 2.0f * m.rgh * map_pow_grad(t[5]);
+    // normal: n = vec/|vec|, vec = (nx, ny, sqrt(1 - clamp(nx^2+ny^2, 0, 1-eps)))
+    const float nx = tclamp(t[3], -1.0f, 1.0f);
+    const float ny = tclamp(t[4], -1.0f, 1.0f);
+    const float s_raw = nx * nx + ny * ny;
+    const float s = tclamp(s_raw, 0.0f, kOneMinusEps);
+    const float nz = fsqrt(1.0f - s);
+    const float il = frsq(nx * nx + ny * ny + nz * nz);
+    const float il3 = il * il * il;
+    const float proj = a.gn[0] * nx + a.gn[1] * ny + a.gn[2] * nz;
+    const float gvx = a.gn[0] * il - proj * nx * il3;
+    const float gvy = a.gn[1] * il - proj * ny * il3;
+    const float gvz = a.gn[2] * il - proj * nz * il3;
+    float gs = -gvz * 0.5f * frcp(nz);
+    gs = (s_raw >= 0.0f && s_raw <= kOneMinusEps) ? gs : 0.0f;
+    const float gnx = gvx + gs * 2.0f * nx;
+    const float gny = gvy + gs * 2.0f * ny;
+    gt[3] = (t[3] >= -1.0f && t[3] <= 1.0f) ? gnx : 0.0f;
+    gt[4] = (t[4] >= -1.0f && t[4] <= 1.0f) ? gny : 0.0f;
+}
+
+}  // namespace svbrdf

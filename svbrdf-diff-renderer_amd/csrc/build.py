@@ -1,0 +1,55 @@
+"""Builds libsvbrdf_hip.so (gfx950) next to the package.
+
+    python svbrdf-diff-renderer_amd/csrc/build.py [--asm]
+
+hipcc cross-compiles without a GPU.  ``--asm`` also writes the gfx950
+assembly of the device code to csrc/_build/ for instruction counting.
+"""
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+
+CSRC = pathlib.Path(__file__).resolve().parent
+PKG = CSRC.parent
+ROOT = PKG.parent
+LIB = PKG / "libsvbrdf_hip.so"
+SOURCES = [CSRC / "svbrdf_hip.hip"]
+HEADERS = [CSRC / "brdf.hpp", ROOT / "include" / "svbrdf_hip.h"]
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # packed FP32 instructions stay off (hardware notes)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-command-line-argument",
+    f"-I{ROOT / 'include'}",
+]
+
+
+def build(asm=False, verbose=False):
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, *FLAGS, *map(str, SOURCES), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    if asm:
+        out = CSRC / "_build"
+        out.mkdir(exist_ok=True)
+        subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-S",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        f"-I{ROOT / 'include'}", str(SOURCES[0]), "-o", str(out / "svbrdf_hip.s")],
+                       check=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(asm="--asm" in sys.argv, verbose=True)
+    print("built", LIB)

@@ -1,0 +1,244 @@
+"""Per-pixel SVBRDF optimisation on the fused gfx950 step kernel.
+
+``SvbrdfOptim`` mirrors the reference optimiser (src/svbrdf.py:15-83 on top
+of src/optimization.py:10-42): same constructor, initialisers, target
+loading and ``optim(epochs, lr, svbrdf_obj, optim_light)`` loop, with the
+loop body -- clamp, render, loss, backward, Adam -- executed as ONE kernel
+launch per iteration (:class:`FusedStep`).
+
+``FusedStep`` owns the device-resident pieces of one optimisation problem:
+the prepared targets, Adam's moment buffers, the step state and the
+workspace for per-block partial sums.  Its ``step()`` is graph-capturable
+(no host scalars change between iterations); ``capture()`` records one
+iteration in a HIP graph and ``replay()`` runs it.
+"""
+import ctypes
+import datetime
+import pathlib
+
+import numpy as np
+import torch
+
+from . import _lib
+
+LOSSES = {"l2": _lib.LOSS_L2, "log_l2": _lib.LOSS_LOG_L2, "logl2": _lib.LOSS_LOG_L2}
+
+# layout of svbrdf_step_state (include/svbrdf_hip.h)
+_STATE_DTYPE = np.dtype([("bias_correction1", "<f8"), ("bias_correction2", "<f8"), ("step", "<f4"),
+                         ("loss", "<f4"), ("blocks_done", "<u4"), ("n_partials", "<u4"),
+                         ("light_exp_avg", "<f4", 4), ("light_exp_avg_sq", "<f4", 4),
+                         ("light_grad", "<f4", 4)])
+
+
+def loss_mode(loss):
+    if isinstance(loss, int):
+        return loss
+    try:
+        return LOSSES[loss.lower()]
+    except KeyError:
+        raise ValueError(f"unknown loss {loss!r}; expected one of {sorted(LOSSES)}") from None
+
+
+class FusedStep:
+    """One optimisation problem: textures (updated in place) against targets.
+
+    textures: [1,9,R,R] or [9,R,R] float32 CUDA tensor, updated in place.
+    targets:  [V,3,R,R] float32 images in gamma space (what Microfacet.eval
+              produces and what src/svbrdf.py:191-204 loads).
+    renderer: a :class:`Microfacet` (geometry, light power, size).
+    """
+
+    def __init__(self, renderer, textures, targets, lr, loss="l2", betas=(0.9, 0.999), eps=1e-8,
+                 optim_light=False, history=0):
+        self.renderer = renderer
+        self.res, self.n_views = renderer.res, renderer.n_of_imgs
+        dev = renderer.device
+        if textures.device.type != "cuda" or textures.dtype != torch.float32 or not textures.is_contiguous():
+            raise ValueError("textures must be a contiguous float32 CUDA tensor")
+        if textures.numel() != 9 * self.res * self.res:
+            raise ValueError(f"textures must hold 9x{self.res}x{self.res} values, got {tuple(textures.shape)}")
+        if tuple(targets.shape) != (self.n_views, 3, self.res, self.res):
+            raise ValueError(f"targets must be [{self.n_views}, 3, {self.res}, {self.res}], "
+                             f"got {tuple(targets.shape)}")
+        self.textures = textures
+        self.lr = float(lr)
+        self.beta1, self.beta2 = float(betas[0]), float(betas[1])
+        self.eps = float(eps)
+        self.mode = loss_mode(loss)
+        self.optim_light = bool(optim_light)
+        if self.optim_light:
+            lp = renderer.light_pow
+            if lp.device != dev or lp.dtype != torch.float32 or not lp.is_contiguous() or lp.numel() != 3:
+                raise ValueError("optim_light needs renderer.light_pow as a contiguous float32 [3] CUDA tensor")
+            self.light_pow = lp
+        else:
+            self.light_pow = renderer.light_pow.detach().to(torch.float32).reshape(3).contiguous()
+        self.exp_avg = torch.zeros_like(textures)
+        self.exp_avg_sq = torch.zeros_like(textures)
+        self.targets = torch.empty_like(targets, dtype=torch.float32, device=dev).contiguous()
+        tg = targets.detach().to(device=dev, dtype=torch.float32).contiguous()
+        s = _lib.stream_handle(dev)
+        _lib.call("svbrdf_prepare_targets", _lib.ptr(tg), tg.numel(), self.mode, _lib.ptr(self.targets), s)
+        nstate = _lib.query("svbrdf_step_state_bytes")
+        self.state = torch.zeros((nstate + 15) // 16 * 4, dtype=torch.float32, device=dev)
+        nws = _lib.query("svbrdf_fused_step_workspace_bytes", self.res)
+        self.workspace = torch.empty((nws + 15) // 16 * 4, dtype=torch.float32, device=dev)
+        self.history = torch.zeros(max(int(history), 0), dtype=torch.float32, device=dev)
+        _lib.call("svbrdf_step_state_init", _lib.ptr(self.state), self.beta1, self.beta2, s)
+        self.steps_launched = 0
+        self.graph = None
+
+    def step(self):
+        """Launch one fused iteration on the current stream (asynchronous)."""
+        r = self.renderer
+        hist = self.history if self.history.numel() else None
+        _lib.call("svbrdf_fused_step", _lib.ptr(self.textures), _lib.ptr(self.exp_avg), _lib.ptr(self.exp_avg_sq),
+                  _lib.ptr(self.targets), self.res, self.n_views, r.size, _lib.ptr(r.camera_pos),
+                  _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode, int(self.optim_light), self.lr,
+                  self.beta1, self.beta2, self.eps, _lib.ptr(self.state), _lib.ptr(hist),
+                  self.history.numel(), _lib.ptr(self.workspace), self.workspace.numel() * 4,
+                  _lib.stream_handle(self.textures.device))
+        self.steps_launched += 1
+
+    def capture(self):
+        """Record one iteration into a HIP graph (torch.cuda.CUDAGraph)."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=self.textures.device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            g.capture_begin()
+            r = self.renderer
+            hist = self.history if self.history.numel() else None
+            _lib.call("svbrdf_fused_step", _lib.ptr(self.textures), _lib.ptr(self.exp_avg),
+                      _lib.ptr(self.exp_avg_sq), _lib.ptr(self.targets), self.res, self.n_views, r.size,
+                      _lib.ptr(r.camera_pos), _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode,
+                      int(self.optim_light), self.lr, self.beta1, self.beta2, self.eps, _lib.ptr(self.state),
+                      _lib.ptr(hist), self.history.numel(), _lib.ptr(self.workspace),
+                      self.workspace.numel() * 4, ctypes.c_void_p(s.cuda_stream))
+            g.capture_end()
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        self.steps_launched += 1
+
+    def read_state(self):
+        """Host copy of the device step state (synchronises)."""
+        raw = self.state.detach().cpu().numpy().view(np.uint8)[:_STATE_DTYPE.itemsize]
+        return raw.view(_STATE_DTYPE)[0]
+
+    def loss(self):
+        """Loss evaluated by the last completed step (synchronises)."""
+        return float(self.read_state()["loss"])
+
+    def losses(self):
+        """Per-step losses recorded so far (synchronises)."""
+        n = min(self.steps_launched, self.history.numel())
+        return self.history[:n].cpu().numpy()
+
+    def state_dict(self):
+        """Exact resume point: maps, Adam moments and the step state."""
+        return {"textures": self.textures.detach().clone(), "exp_avg": self.exp_avg.clone(),
+                "exp_avg_sq": self.exp_avg_sq.clone(), "state": self.state.clone(),
+                "light_pow": self.light_pow.detach().clone()}
+
+    def load_state_dict(self, sd):
+        with torch.no_grad():
+            self.textures.copy_(sd["textures"])
+            self.exp_avg.copy_(sd["exp_avg"])
+            self.exp_avg_sq.copy_(sd["exp_avg_sq"])
+            self.state.copy_(sd["state"])
+            if self.optim_light:
+                self.light_pow.copy_(sd["light_pow"])
+
+
+class SvbrdfOptim:
+    """src/svbrdf.py:15-83 (and the Optim base, src/optimization.py:10-42)."""
+
+    def __init__(self, device, renderer_obj):
+        self.device = torch.device(device)
+        self.eps = 1e-4
+        self.renderer_obj = renderer_obj
+        self.res = renderer_obj.res
+        self.loss = "l2"
+        self.fused = None
+
+    # --- initialisers: src/svbrdf.py:20-39 (same encodings, t = 2u - 1) ---
+    def init_from_tex(self, textures):
+        self.textures = textures.detach().to(self.device, torch.float32).contiguous().requires_grad_(True)
+
+    def init_from_const(self, dif=0.5, spe=0.04, rgh=0.2):
+        R, d = self.res, self.device
+        normal = torch.zeros(1, 2, R, R, device=d)
+        diffuse = torch.ones(1, 3, R, R, device=d) * dif * 2 - 1
+        specular = torch.ones(1, 3, R, R, device=d) * spe * 2 - 1
+        roughness = torch.ones(1, 1, R, R, device=d) * rgh * 2 - 1
+        self.init_from_tex(torch.cat((diffuse, normal, roughness, specular), 1))
+
+    def init_from_randn(self, dif=0.5, spe=0.04, rgh=0.2):
+        R, d = self.res, self.device
+        normal = (torch.randn(1, 2, R, R, device=d) / 4).clamp(-1, 1)
+        diffuse = (torch.randn(1, 3, R, R, device=d) / 8 + dif).clamp(0, 1) * 2 - 1
+        specular = (torch.randn(1, 3, R, R, device=d) / 32 + spe).clamp(0, 1) * 2 - 1
+        roughness = (torch.randn(1, 1, R, R, device=d) / 16 + rgh).clamp(0, 1) * 2 - 1
+        self.init_from_tex(torch.cat((diffuse, normal, roughness, specular), 1))
+
+    def load_targets(self, targets):
+        self.targets = targets
+
+    def compute_image_loss(self, predicts):
+        """src/optimization.py:28-29 (MSE); "log_l2" when self.loss says so."""
+        if loss_mode(self.loss) == _lib.LOSS_L2:
+            return torch.nn.functional.mse_loss(predicts, self.targets)
+        floor = 1e-6 ** (1 / 2.2)
+        return torch.nn.functional.mse_loss(torch.log(predicts), torch.log(self.targets.clamp(floor, 1)))
+
+    def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False, loss=None, save_every=100,
+              sync_every=0):
+        """src/svbrdf.py:44-83.
+
+        The loop body runs as one fused kernel per epoch.  Side outputs are as
+        in the reference when ``svbrdf_obj`` provides them (maps and
+        re-renders saved at epoch 1, every ``save_every`` epochs and the last
+        epoch under optimize_dir/tmp/<timestamp>/<epoch>/).  Losses are
+        recorded on the device and read back at the end (or every
+        ``sync_every`` epochs), instead of with one ``.item()`` per epoch.
+        """
+        if loss is not None:
+            self.loss = loss
+        r = self.renderer_obj
+        if optim_light:
+            if svbrdf_obj is not None:
+                lp = svbrdf_obj.cl[2]
+            else:
+                lp = r.light_pow
+            lp = lp.detach().to(self.device, torch.float32).reshape(3).contiguous()
+            if svbrdf_obj is not None:
+                svbrdf_obj.cl[2] = lp
+            r.update_light(lp)
+        with torch.no_grad():
+            tex = self.textures.detach()
+        self.fused = FusedStep(r, tex, self.targets, lr, loss=self.loss, optim_light=optim_light,
+                               history=epochs)
+        tmp_dir = None
+        if svbrdf_obj is not None and hasattr(svbrdf_obj, "optimize_dir"):
+            stamp = str(datetime.datetime.now()).replace(" ", "-").replace(":", "-").replace(".", "-")
+            tmp_dir = pathlib.Path(svbrdf_obj.optimize_dir) / "tmp" / stamp
+            tmp_dir.mkdir(parents=True, exist_ok=True)
+        for epoch in range(epochs):
+            self.fused.step()
+            if sync_every and (epoch + 1) % sync_every == 0:
+                torch.cuda.synchronize(self.device)
+            if tmp_dir is not None and ((epoch + 1) % save_every == 0 or epoch == 0 or epoch == epochs - 1):
+                this_dir = tmp_dir / f"{epoch + 1}"
+                this_dir.mkdir(parents=True, exist_ok=True)
+                with torch.no_grad():
+                    tc = self.textures.detach().clamp(-1, 1)
+                    if hasattr(svbrdf_obj, "save_textures_th"):
+                        svbrdf_obj.save_textures_th(tc, this_dir)
+                    if hasattr(svbrdf_obj, "save_images_th"):
+                        svbrdf_obj.save_images_th(r.eval(tc), this_dir)
+        self.loss_image_list = [float(x) for x in self.fused.losses()]
+        return self.loss_image_list

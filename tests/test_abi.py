@@ -1,0 +1,118 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point
+include/svbrdf_hip.h declares, and rejects bad arguments on the host (before
+any HIP call) with the documented status codes."""
+import ctypes
+import re
+
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER = ROOT / "include" / "svbrdf_hip.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(svbrdf_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib(pkg):
+    return pkg._lib.load()
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for n in ("svbrdf_render", "svbrdf_render_backward", "svbrdf_fused_step", "svbrdf_prepare_targets",
+              "svbrdf_step_state_init", "svbrdf_abi_version", "svbrdf_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(lib, pkg):
+    for n in declared_functions():
+        assert hasattr(lib, n), n
+        assert n in pkg._lib.SIGNATURES, f"{n} has no ctypes signature"
+
+
+def test_queries(lib, pkg):
+    assert lib.svbrdf_abi_version() == 1
+    # the host-side view of svbrdf_step_state matches the C struct
+    assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 80
+    assert lib.svbrdf_max_res() >= 4096
+    assert lib.svbrdf_max_views() >= 16
+    # one float4 partial per 256-pixel block
+    assert lib.svbrdf_fused_step_workspace_bytes(1024) == 1024 * 1024 // 256 * 16
+    assert lib.svbrdf_fused_step_workspace_bytes(0) == 0
+    assert lib.svbrdf_render_backward_workspace_bytes(13) == 1 * 16
+
+
+def _fake(n=1):
+    # never dereferenced: validation fails before any launch
+    return ctypes.c_void_p(0x1000 * n)
+
+
+@pytest.mark.parametrize("res,views,code", [(0, 9, -2), (-4, 9, -2), (1 << 20, 9, -2), (8, 0, -2), (8, 100000, -2)])
+def test_render_rejects_bad_shapes(lib, res, views, code):
+    rc = lib.svbrdf_render(_fake(1), res, views, ctypes.c_float(17.0), _fake(2), _fake(3), _fake(4), _fake(5), None)
+    assert rc == code
+    assert b"svbrdf_render" in lib.svbrdf_last_error()
+
+
+def test_render_rejects_null_pointers(lib):
+    assert lib.svbrdf_render(None, 8, 9, ctypes.c_float(17.0), _fake(2), _fake(3), _fake(4), _fake(5), None) == -1
+    assert lib.svbrdf_render(_fake(1), 8, 9, ctypes.c_float(17.0), _fake(2), _fake(3), _fake(4), None, None) == -1
+
+
+def test_backward_requires_workspace_for_light_gradient(lib):
+    rc = lib.svbrdf_render_backward(_fake(1), 64, 9, ctypes.c_float(17.0), _fake(2), _fake(3), _fake(4), _fake(5),
+                                    _fake(6), _fake(7), None, 0, None)
+    assert rc == -3
+    assert b"workspace" in lib.svbrdf_last_error()
+
+
+def _step(lib, **kw):
+    a = dict(tex=_fake(1), m=_fake(2), v=_fake(3), tgt=_fake(4), res=64, views=9, cam=_fake(5), light=_fake(6),
+             pow_=_fake(7), mode=1, light_opt=0, lr=0.01, b1=0.9, b2=0.999, eps=1e-8, state=_fake(8),
+             hist=None, hlen=0, ws=_fake(9), wsb=64 * 64 // 256 * 16)
+    a.update(kw)
+    return lib.svbrdf_fused_step(a["tex"], a["m"], a["v"], a["tgt"], a["res"], a["views"], ctypes.c_float(17.0),
+                                 a["cam"], a["light"], a["pow_"], a["mode"], a["light_opt"],
+                                 ctypes.c_double(a["lr"]), ctypes.c_double(a["b1"]), ctypes.c_double(a["b2"]),
+                                 ctypes.c_double(a["eps"]), a["state"], a["hist"], a["hlen"], a["ws"], a["wsb"], None)
+
+
+def test_fused_step_validation(lib):
+    assert _step(lib, mode=7) == -1
+    assert _step(lib, b1=1.0) == -1
+    assert _step(lib, lr=float("nan")) == -1
+    assert _step(lib, state=None) == -1
+    assert _step(lib, wsb=16) == -3
+    assert _step(lib, ws=ctypes.c_void_p(0x1004)) == -1      # misaligned workspace
+    assert _step(lib, res=0) == -2
+
+
+def test_prepare_and_state_validation(lib):
+    assert lib.svbrdf_prepare_targets(None, 10, 1, _fake(1), None) == -1
+    assert lib.svbrdf_prepare_targets(_fake(1), 10, 5, _fake(1), None) == -1
+    assert lib.svbrdf_prepare_targets(_fake(1), 0, 1, _fake(1), None) == 0     # nothing to do
+    assert lib.svbrdf_step_state_init(None, 0.9, 0.999, None) == -1
+    assert lib.svbrdf_step_state_init(_fake(1), 0.9, 1.5, None) == -1
+
+
+def test_python_wrapper_refuses_cpu(pkg):
+    import torch
+    cl = [torch.zeros(9, 3), torch.zeros(9, 3), torch.ones(3)]
+    with pytest.raises(RuntimeError, match="GPU only"):
+        pkg.Microfacet(8, 9, 17.0, cl, "cpu")
+
+
+def test_python_wrapper_keeps_reference_asserts(pkg):
+    """Microfacet.eval asserts square, res-matching textures
+    (src/microfacet.py:85-86) before any device work."""
+    import torch
+    r = pkg.Microfacet.__new__(pkg.Microfacet)
+    r.res = 8
+    with pytest.raises(AssertionError):
+        r.eval(torch.zeros(1, 9, 8, 6))
+    with pytest.raises(AssertionError):
+        r.eval(torch.zeros(1, 9, 6, 6))
