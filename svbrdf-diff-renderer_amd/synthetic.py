@@ -37,6 +37,8 @@ def height_field_normals(res, rng, sigma_px=None, amplitude=0.5):
     """(nx, ny) of a Gaussian-filtered white-noise height field, float32."""
     if sigma_px is None:
         sigma_px = 4.0 * res / 256.0
+    if res < 2:
+        return np.zeros((res, res), np.float32), np.zeros((res, res), np.float32)
     h = _gaussian_filter_periodic(rng.standard_normal((res, res)), sigma_px)
     h *= amplitude / max(np.abs(h).max(), 1e-30)
     dy, dx = np.gradient(h)

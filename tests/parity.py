@@ -7,7 +7,7 @@ Forward images (FP32):
   the GPU's max relative error against the FP64 render is at most twice the
   reference FP32's own (+1e-6).  On well-conditioned pixels sigma ~ 0 and the
   bound is the plain 1e-5 relative / 1e-6 absolute; additionally at least
-  99.5 % of all elements must meet that plain bound.
+  99.9 % of all elements must meet that plain bound against the FP64 render.
 
 Gradients (FP32), per texture channel:
   ||g_gpu - g64|| <= max(1e-4 ||g64||, 2 ||g_ref32 - g64||): within 1e-4
@@ -27,11 +27,13 @@ def check_forward(out, ref32, ref64, sigma, label=""):
     assert not bad.any(), (f"{label}: {int(bad.sum())} / {bad.size} elements out of tolerance; worst excess "
                            f"{(err - bound).max():.3e} at {np.unravel_index((err - bound).argmax(), err.shape)}")
     frac_plain = float(np.mean(err <= plain))
-    assert frac_plain >= 0.995, f"{label}: only {frac_plain:.5f} of elements within the plain 1e-5 bound"
+    frac64 = float(np.mean(np.abs(out - ref64) <= 1e-6 + 1e-5 * np.abs(ref64)))
+    assert frac64 >= 0.999, f"{label}: only {frac64:.5f} of elements within 1e-5 rel / 1e-6 abs of FP64"
     e_gpu = float((np.abs(out - ref64) / ref64).max())
     e_ref = float((np.abs(ref32 - ref64) / ref64).max())
     assert e_gpu <= 2 * e_ref + 1e-6, f"{label}: max rel err vs FP64 {e_gpu:.3e} > 2 x reference FP32's {e_ref:.3e}"
-    return dict(max_rel_vs_fp64=e_gpu, ref32_max_rel_vs_fp64=e_ref, frac_plain=frac_plain)
+    return dict(max_rel_vs_fp64=e_gpu, ref32_max_rel_vs_fp64=e_ref, frac_plain_vs_ref32=frac_plain,
+                frac_plain_vs_fp64=frac64)
 
 
 def check_gradient(g, g32, g64, label=""):
