@@ -204,6 +204,9 @@ def main():
     ap.add_argument("--no-torch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the bench, run 5 plain device copies of MIB MiB (a known-size kernel that "
+                         "calibrates the profiler's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (see DESIGN.md, sharding)",
@@ -266,6 +269,14 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(res, views, args.cpu_seconds)
         out["cpu_baseline"] = cb
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = n * 4
     print(json.dumps(out), flush=True)
 
 

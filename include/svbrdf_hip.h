@@ -54,10 +54,11 @@ enum svbrdf_loss {
 typedef struct svbrdf_step_state {
     double bias_correction1;     /* 1 - beta1^t for the step the next call performs */
     double bias_correction2;     /* 1 - beta2^t                                     */
+    double inv_bias_correction1; /* 1 / bias_correction1                            */
+    float inv_bias_correction2_sqrt; /* 1 / sqrt(bias_correction2), rounded to fp32 */
     float step;                  /* completed steps (torch keeps it as a float)     */
     float loss;                  /* image loss evaluated by the last completed step */
-    unsigned int blocks_done;    /* internal: blocks finished in the running call   */
-    unsigned int n_partials;     /* internal: partial sums written by the last call */
+    unsigned int reserved[3];
     float light_exp_avg[4];      /* Adam state of light_pow (when optimised)        */
     float light_exp_avg_sq[4];
     float light_grad[4];         /* dL/dlight_pow of the last step (when optimised) */
@@ -116,7 +117,9 @@ int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream
  * history_len, also in loss_history[t] (device memory), so a loop can keep
  * every loss without a host synchronisation per iteration (the reference
  * reads loss.item() each epoch, src/svbrdf.py:64).  The workspace holds
- * per-block partial sums (svbrdf_fused_step_workspace_bytes(res) bytes). */
+ * per-block partial sums (svbrdf_fused_step_workspace_bytes(res) bytes).
+ * The call enqueues two kernels: the per-pixel step and a one-block
+ * finalize that reduces the partial sums and advances `state`. */
 size_t svbrdf_fused_step_workspace_bytes(int res);
 int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq,
                       const float *targets_prepared, int res, int n_views, float size,

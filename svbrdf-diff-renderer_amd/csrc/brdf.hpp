@@ -98,19 +98,19 @@ __device__ __forceinline__ void decode(const float t[9], Maps &m) {
 struct Geom {
     float v[3], l[3], h[3];
     float inv_d2, vh;
-    bool coloc;
 };
 
-// `coloc` (camera == light for this view, a wave-uniform fact) makes l = v and
-// h = v/|v| exactly as the general formulas would compute them, with fewer ops.
+// COLOC (camera == light, true for every view of a launch, decided on the
+// device before the view loop) makes l = v and h = v/|v| exactly as the
+// general formulas would compute them, with fewer ops.
+template <bool COLOC>
 __device__ __forceinline__ void geometry(float px, float py, float cx, float cy, float cz,
-                                         float lx, float ly, float lz, bool coloc, Geom &g) {
+                                         float lx, float ly, float lz, Geom &g) {
     const float vx = cx - px, vy = cy - py, vz = cz;
     const float sv = vx * vx + vy * vy + vz * vz;
     const float rv = frsq(sv);
     g.v[0] = vx * rv; g.v[1] = vy * rv; g.v[2] = vz * rv;
-    g.coloc = coloc;
-    if (coloc) {
+    if (COLOC) {
         g.l[0] = g.v[0]; g.l[1] = g.v[1]; g.l[2] = g.v[2];
         g.inv_d2 = frcp(sv);
         const float w = g.v[0] * g.v[0] + g.v[1] * g.v[1] + g.v[2] * g.v[2];
@@ -139,9 +139,10 @@ struct Shade {
     float nl_d2, f[3], img[3];
 };
 
+template <bool COLOC>
 __device__ __forceinline__ void shade(const Maps &m, const Geom &g, const float P[3], Shade &s) {
     s.nv_raw = m.n[0] * g.v[0] + m.n[1] * g.v[1] + m.n[2] * g.v[2];
-    if (g.coloc) {
+    if (COLOC) {
         s.nl_raw = s.nv_raw;
     } else {
         s.nl_raw = m.n[0] * g.l[0] + m.n[1] * g.l[1] + m.n[2] * g.l[2];
@@ -168,7 +169,7 @@ __device__ __forceinline__ void shade(const Maps &m, const Geom &g, const float 
     // Smith(n.v, n.l, rgh^2)
     s.idv = frcp(s.nv * m.omk + m.k);
     s.g1v = s.nv * s.idv;
-    if (g.coloc) {
+    if (COLOC) {
         s.idl = s.idv;
         s.g1l = s.g1v;
     } else {

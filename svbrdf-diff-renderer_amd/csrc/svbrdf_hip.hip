@@ -59,15 +59,15 @@ int check_common(const char *what, const void *tex, int res, int n_views, const 
     return SVBRDF_OK;
 }
 
-inline unsigned n_blocks(int res) {
+inline unsigned n_blocks(int res, int npix_per_thread = 1) {
     const size_t npix = (size_t)res * res;
-    return (unsigned)((npix + kBlock - 1) / kBlock);
+    const size_t per_block = (size_t)kBlock * npix_per_thread;
+    return (unsigned)((npix + per_block - 1) / per_block);
 }
 
 // Geometry of view k, read with wave-uniform addresses (scalar loads).
 struct ViewPos {
     float cx, cy, cz, lx, ly, lz;
-    bool coloc;
 };
 
 __device__ __forceinline__ ViewPos load_view(const float *__restrict__ cam, const float *__restrict__ light,
@@ -75,8 +75,58 @@ __device__ __forceinline__ ViewPos load_view(const float *__restrict__ cam, cons
     ViewPos p;
     p.cx = cam[3 * k + 0]; p.cy = cam[3 * k + 1]; p.cz = cam[3 * k + 2];
     p.lx = light[3 * k + 0]; p.ly = light[3 * k + 1]; p.lz = light[3 * k + 2];
-    p.coloc = (p.cx == p.lx) && (p.cy == p.ly) && (p.cz == p.lz);
     return p;
+}
+
+// True when every view's light sits exactly at its camera (the flash setup of
+// the reference's captures): then the shading uses the co-located forms.
+__device__ __forceinline__ bool all_colocated(const float *__restrict__ cam, const float *__restrict__ light,
+                                              int n_views) {
+    bool all = true;
+    for (int e = 0; e < 3 * n_views; ++e) all = all && (cam[e] == light[e]);
+    return all;
+}
+
+// The pixel a thread owns.  Threads past the end keep working on the last
+// pixel (so every load stays in bounds and no branch splits the wave) but
+// store nothing and contribute nothing to the sums.
+struct Pix {
+    unsigned p;      // clamped pixel index
+    unsigned off;    // its byte offset within a plane (npix <= 2^28, so it fits)
+    bool valid;
+    float px, py;    // position on the plane
+};
+
+// Plane access through buffer instructions: a resource descriptor for plane
+// c of a tensor (base, byte size; four SGPRs, wave-uniform) plus the lane's
+// 32-bit byte offset, which every plane shares.  Offsets past the plane read
+// 0 and drop stores in hardware.  Word 3 = 0x00020000 selects the 32-bit data
+// format that untyped gfx9 buffer accesses need.
+struct Plane {
+    __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Plane plane(const float *base, unsigned c, unsigned npix) {
+    Plane p;
+    p.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base) + (size_t)c * npix, 0, (int)(npix * 4u),
+                                            0x00020000);
+    return p;
+}
+__device__ __forceinline__ float ld(Plane p, unsigned byte_off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p.r, byte_off, 0, 0));
+}
+__device__ __forceinline__ void st(Plane p, unsigned byte_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p.r, byte_off, 0, 0);
+}
+
+__device__ __forceinline__ Pix pixel_at(unsigned p, unsigned npix, int res, float size) {
+    Pix x;
+    x.valid = p < npix;
+    x.p = x.valid ? p : npix - 1u;
+    x.off = x.p * 4u;
+    const unsigned i = x.p / (unsigned)res, j = x.p - i * (unsigned)res;
+    x.px = pixel_coord((int)j, res, size);
+    x.py = -pixel_coord((int)i, res, size);
+    return x;
 }
 
 // Wave64 + block reduction of a float; result valid in thread 0.
@@ -111,79 +161,112 @@ __device__ __forceinline__ double block_sum_d(double x, double *lds) {
 }
 
 // ---------------------------------------------------------------------------
-// Forward render.
-__global__ __launch_bounds__(kBlock) void render_kernel(
-    const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
-    const float *__restrict__ light, const float *__restrict__ lpow, float *__restrict__ out) {
-    const size_t npix = (size_t)res * res;
-    const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npix) return;
-    const int i = (int)(p / (unsigned)res), j = (int)(p - (size_t)i * res);
-    const float px = pixel_coord(j, res, size), py = -pixel_coord(i, res, size);
-    float t[9];
+// Forward render: NPIX pixels per thread (block-strided, so each plane access
+// of a wave stays one contiguous 256-byte run).
+template <bool COLOC, int NPIX>
+__device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float *__restrict__ tex, unsigned npix,
+                                              int n_views, const float *__restrict__ cam,
+                                              const float *__restrict__ light, const float P[3],
+                                              float *__restrict__ out) {
+    Maps m[NPIX];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) t[c] = tex[c * npix + p];
-    Maps m;
-    decode(t, m);
-    const float P[3] = {lpow[0], lpow[1], lpow[2]};
+    for (int q = 0; q < NPIX; ++q) {
+        float t[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = ld(plane(tex, c, npix), px[q].off);
+        decode(t, m[q]);
+    }
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
-        Geom g;
-        geometry(px, py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, vp.coloc, g);
-        Shade s;
-        shade(m, g, P, s);
+        float *ok = out + (size_t)(3 * k) * npix;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float ic = tclamp(s.img[c], kEps, 1.0f);
-            out[(size_t)(3 * k + c) * npix + p] = fexp2(flog2(ic) * kInvGamma);
+        for (int q = 0; q < NPIX; ++q) {
+            Geom g;
+            geometry<COLOC>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
+            Shade s;
+            shade<COLOC>(m[q], g, P, s);
+            if (px[q].valid) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float ic = tclamp(s.img[c], kEps, 1.0f);
+                    st(plane(ok, c, npix), px[q].off, fexp2(flog2(ic) * kInvGamma));
+                }
+            }
         }
     }
 }
 
+template <int NPIX>
+__global__ __launch_bounds__(kBlock) void render_kernel(
+    const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
+    const float *__restrict__ light, const float *__restrict__ lpow, float *__restrict__ out) {
+    const unsigned npix = (unsigned)res * (unsigned)res;
+    Pix px[NPIX];
+#pragma unroll
+    for (int q = 0; q < NPIX; ++q)
+        px[q] = pixel_at(blockIdx.x * (kBlock * NPIX) + q * kBlock + threadIdx.x, npix, res, size);
+    const float P[3] = {lpow[0], lpow[1], lpow[2]};
+    if (all_colocated(cam, light, n_views))
+        render_pixels<true, NPIX>(px, tex, npix, n_views, cam, light, P, out);
+    else
+        render_pixels<false, NPIX>(px, tex, npix, n_views, cam, light, P, out);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint of the render for an arbitrary dL/dimages.
+template <bool COLOC, bool LIGHT_GRAD>
+__device__ __forceinline__ void backward_pixel(const Pix &px, const float *__restrict__ tex, unsigned npix,
+                                               int n_views, const float *__restrict__ cam,
+                                               const float *__restrict__ light, const float P[3],
+                                               const float *__restrict__ gout, float *__restrict__ gtex,
+                                               Accum &a) {
+    float t[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) t[c] = ld(plane(tex, c, npix), px.off);
+    Maps m;
+    decode(t, m);
+    for (int k = 0; k < n_views; ++k) {
+        const ViewPos vp = load_view(cam, light, k);
+        const float *gk = gout + (size_t)(3 * k) * npix;
+        Geom g;
+        geometry<COLOC>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
+        Shade s;
+        shade<COLOC>(m, g, P, s);
+        float gimg[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float img = s.img[c];
+            const float ic = tclamp(img, kEps, 1.0f);
+            const float o = fexp2(flog2(ic) * kInvGamma);
+            const float go = px.valid ? ld(plane(gk, c, npix), px.off) : 0.0f;
+            // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
+            gimg[c] = (img >= kEps && img <= 1.0f) ? go * kInvGamma * o * frcp(ic) : 0.0f;
+        }
+        shade_backward<LIGHT_GRAD>(m, g, s, P, gimg, a);
+    }
+    float gt[9];
+    decode_backward(t, m, a, gt);
+    if (px.valid) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) st(plane(gtex, c, npix), px.off, gt[c]);
+    }
+}
+
 template <bool LIGHT_GRAD>
 __global__ __launch_bounds__(kBlock) void backward_kernel(
     const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, const float *__restrict__ lpow, const float *__restrict__ gout,
     float *__restrict__ gtex, float *__restrict__ light_partials) {
     __shared__ float lds[kBlock / 64];
-    const size_t npix = (size_t)res * res;
-    const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const unsigned npix = (unsigned)res * (unsigned)res;
+    const Pix px = pixel_at(blockIdx.x * kBlock + threadIdx.x, npix, res, size);
+    const float P[3] = {lpow[0], lpow[1], lpow[2]};
     Accum a;
     accum_zero(a);
-    if (p < npix) {
-        const int i = (int)(p / (unsigned)res), j = (int)(p - (size_t)i * res);
-        const float px = pixel_coord(j, res, size), py = -pixel_coord(i, res, size);
-        float t[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) t[c] = tex[c * npix + p];
-        Maps m;
-        decode(t, m);
-        const float P[3] = {lpow[0], lpow[1], lpow[2]};
-        for (int k = 0; k < n_views; ++k) {
-            const ViewPos vp = load_view(cam, light, k);
-            Geom g;
-            geometry(px, py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, vp.coloc, g);
-            Shade s;
-            shade(m, g, P, s);
-            float gimg[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float img = s.img[c];
-                const float ic = tclamp(img, kEps, 1.0f);
-                const float o = fexp2(flog2(ic) * kInvGamma);
-                const float go = gout[(size_t)(3 * k + c) * npix + p];
-                // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
-                gimg[c] = (img >= kEps && img <= 1.0f) ? go * kInvGamma * o * frcp(ic) : 0.0f;
-            }
-            shade_backward<LIGHT_GRAD>(m, g, s, P, gimg, a);
-        }
-        float gt[9];
-        decode_backward(t, m, a, gt);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) gtex[c * npix + p] = gt[c];
-    }
+    if (all_colocated(cam, light, n_views))
+        backward_pixel<true, LIGHT_GRAD>(px, tex, npix, n_views, cam, light, P, gout, gtex, a);
+    else
+        backward_pixel<false, LIGHT_GRAD>(px, tex, npix, n_views, cam, light, P, gout, gtex, a);
     if (LIGHT_GRAD) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -217,14 +300,23 @@ __global__ __launch_bounds__(kBlock) void prepare_targets_kernel(const float *__
     }
 }
 
+// Bias corrections of step t (torch computes them in double from the float
+// step count, torch/optim/adam.py:519-524) and the per-launch forms the
+// kernel reads: 1/bc1 in double (lr / bc1 = lr * (1/bc1) to the last bit of
+// the fp32 result) and 1/sqrt(bc2) in fp32.
+__device__ void set_bias_corrections(svbrdf_step_state *st, double beta1, double beta2, double t) {
+    st->bias_correction1 = 1.0 - pow(beta1, t);
+    st->bias_correction2 = 1.0 - pow(beta2, t);
+    st->inv_bias_correction1 = 1.0 / st->bias_correction1;
+    st->inv_bias_correction2_sqrt = (float)(1.0 / sqrt(st->bias_correction2));
+}
+
 __global__ void state_init_kernel(svbrdf_step_state *st, double beta1, double beta2) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    st->bias_correction1 = 1.0 - beta1;
-    st->bias_correction2 = 1.0 - beta2;
+    set_bias_corrections(st, beta1, beta2, 1.0);
     st->step = 0.0f;
     st->loss = 0.0f;
-    st->blocks_done = 0u;
-    st->n_partials = 0u;
+    st->reserved[0] = st->reserved[1] = st->reserved[2] = 0u;
     for (int c = 0; c < 4; ++c) {
         st->light_exp_avg[c] = 0.0f;
         st->light_exp_avg_sq[c] = 0.0f;
@@ -253,9 +345,9 @@ __device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st,
     s.w1 = (float)(1.0 - beta1);
     s.beta2 = (float)beta2;
     s.om_beta2 = (float)(1.0 - beta2);
-    s.inv_bc2_sqrt = 1.0f / (float)sqrt(st->bias_correction2);
+    s.inv_bc2_sqrt = st->inv_bias_correction2_sqrt;
     s.eps = (float)eps;
-    s.neg_step = (float)(-(lr / st->bias_correction1));
+    s.neg_step = (float)(-lr * st->inv_bias_correction1);
     return s;
 }
 
@@ -263,129 +355,179 @@ __device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st,
 // Fused per-pixel optimisation step.  Every block writes its partial loss (and
 // light-power gradient); the last block to finish reduces them in a fixed
 // order, updates light_pow when it is optimised, and advances the step state.
+#ifndef SVBRDF_FUSED_NPIX
+#define SVBRDF_FUSED_NPIX 1       // pixels per thread
+#endif
+#ifndef SVBRDF_FUSED_UNROLL
+#define SVBRDF_FUSED_UNROLL 1     // views per loop trip
+#endif
+#ifndef SVBRDF_FUSED_WAVES
+#define SVBRDF_FUSED_WAVES 1      // minimum waves per SIMD the register allocator must allow
+#endif
+
+// Loss residual and dL/dimg of one channel: r = ln out - ln target' (log-L2)
+// or out - target (L2), with torch's inclusive clamp mask on img.
+template <int LOSS>
+__device__ __forceinline__ float loss_channel(float img, float y, float K, bool valid, float &loss) {
+    const float ic = tclamp(img, kEps, 1.0f);
+    const float lic = flog2(ic);
+    const bool inside = valid && (img >= kEps && img <= 1.0f);
+    float r, gi;
+    if (LOSS == SVBRDF_LOSS_LOG_L2) {
+        r = lic * (kInvGamma * kLn2) - y;
+        gi = K * kInvGamma * r * frcp(ic);
+    } else {
+        const float o = fexp2(lic * kInvGamma);
+        r = o - y;
+        gi = K * kInvGamma * r * o * frcp(ic);
+    }
+    loss += valid ? r * r : 0.0f;
+    return inside ? gi : 0.0f;
+}
+
+template <int LOSS, bool OPT_LIGHT, bool COLOC, int NPIX>
+__device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__restrict__ tex,
+                                             float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
+                                             const float *__restrict__ tgt, unsigned npix, int n_views,
+                                             const float *__restrict__ cam, const float *__restrict__ light,
+                                             const float P[3], float K, const StepScalars &sc, float &loss,
+                                             float (&gP)[3]) {
+    float traw[NPIX][9];
+    Maps m[NPIX];
+    Accum a[NPIX];
+#pragma unroll
+    for (int q = 0; q < NPIX; ++q) {
+        float t[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            traw[q][c] = ld(plane(tex, c, npix), px[q].off);
+            t[c] = tclamp(traw[q][c], -1.0f, 1.0f);   // textures.clamp(-1, 1), svbrdf.py:60
+        }
+        decode(t, m[q]);
+        accum_zero(a[q]);
+    }
+#pragma unroll SVBRDF_FUSED_UNROLL
+    for (int k = 0; k < n_views; ++k) {
+        const ViewPos vp = load_view(cam, light, k);
+        const float *tk = tgt + (size_t)(3 * k) * npix;
+#pragma unroll
+        for (int q = 0; q < NPIX; ++q) {
+            const float y0 = ld(plane(tk, 0, npix), px[q].off), y1 = ld(plane(tk, 1, npix), px[q].off),
+                        y2 = ld(plane(tk, 2, npix), px[q].off);
+            Geom g;
+            geometry<COLOC>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
+            Shade s;
+            shade<COLOC>(m[q], g, P, s);
+            float gimg[3];
+            gimg[0] = loss_channel<LOSS>(s.img[0], y0, K, px[q].valid, loss);
+            gimg[1] = loss_channel<LOSS>(s.img[1], y1, K, px[q].valid, loss);
+            gimg[2] = loss_channel<LOSS>(s.img[2], y2, K, px[q].valid, loss);
+            shade_backward<OPT_LIGHT>(m[q], g, s, P, gimg, a[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NPIX; ++q) {
+        float t[9], gt[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = tclamp(traw[q][c], -1.0f, 1.0f);
+        decode_backward(t, m[q], a[q], gt);
+        if (OPT_LIGHT) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gP[c] += a[q].gP[c];
+        }
+        if (!px[q].valid) continue;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const unsigned o = px[q].off;
+            float pp = traw[q][c], mm = ld(plane(exp_avg, c, npix), o), vv = ld(plane(exp_avg_sq, c, npix), o);
+            // the clamp on read passes the gradient only inside [-1, 1] (inclusive)
+            const float g = (traw[q][c] >= -1.0f && traw[q][c] <= 1.0f) ? gt[c] : 0.0f;
+            adam_update(pp, mm, vv, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
+            st(plane(tex, c, npix), o, pp);
+            st(plane(exp_avg, c, npix), o, mm);
+            st(plane(exp_avg_sq, c, npix), o, vv);
+        }
+    }
+}
+
 template <int LOSS, bool OPT_LIGHT>
-__global__ __launch_bounds__(kBlock) void fused_step_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_FUSED_WAVES))) void fused_step_kernel(
     float *__restrict__ tex, float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
     const float *__restrict__ tgt, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, float *__restrict__ lpow, double lr, double beta1, double beta2,
-    double eps, svbrdf_step_state *__restrict__ st, float *__restrict__ loss_history, int history_len,
-    float *__restrict__ partials) {
+    double eps, const svbrdf_step_state *__restrict__ st, float *__restrict__ partials) {
+    constexpr int NPIX = SVBRDF_FUSED_NPIX;
     __shared__ float lds[kBlock / 64];
-    __shared__ double ldsd[kBlock / 64];
-    __shared__ int am_last;
-    const size_t npix = (size_t)res * res;
-    const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const unsigned npix = (unsigned)res * (unsigned)res;
     const float K = 2.0f / (float)((double)n_views * 3.0 * (double)npix);   // d mean / d elem
     const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
+    Pix px[NPIX];
+#pragma unroll
+    for (int q = 0; q < NPIX; ++q)
+        px[q] = pixel_at(blockIdx.x * (kBlock * NPIX) + q * kBlock + threadIdx.x, npix, res, size);
     float loss = 0.f;
-    Accum a;
-    accum_zero(a);
-    if (p < npix) {
-        const int i = (int)(p / (unsigned)res), j = (int)(p - (size_t)i * res);
-        const float px = pixel_coord(j, res, size), py = -pixel_coord(i, res, size);
-        float traw[9], t[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            traw[c] = tex[c * npix + p];
-            t[c] = tclamp(traw[c], -1.0f, 1.0f);   // textures.clamp(-1, 1), svbrdf.py:60
-        }
-        Maps m;
-        decode(t, m);
-        for (int k = 0; k < n_views; ++k) {
-            const ViewPos vp = load_view(cam, light, k);
-            Geom g;
-            geometry(px, py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, vp.coloc, g);
-            Shade s;
-            shade(m, g, P, s);
-            float gimg[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float img = s.img[c];
-                const float ic = tclamp(img, kEps, 1.0f);
-                const float lic = flog2(ic);
-                const float y = tgt[(size_t)(3 * k + c) * npix + p];
-                const bool inside = (img >= kEps && img <= 1.0f);
-                float r, gi;
-                if (LOSS == SVBRDF_LOSS_LOG_L2) {
-                    r = lic * (kInvGamma * kLn2) - y;           // ln out - ln target'
-                    gi = K * kInvGamma * r * frcp(ic);
-                } else {
-                    const float o = fexp2(lic * kInvGamma);
-                    r = o - y;
-                    gi = K * kInvGamma * r * o * frcp(ic);
-                }
-                loss += r * r;
-                gimg[c] = inside ? gi : 0.0f;
-            }
-            shade_backward<OPT_LIGHT>(m, g, s, P, gimg, a);
-        }
-        float gt[9];
-        decode_backward(t, m, a, gt);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const size_t e = c * npix + p;
-            float pp = traw[c], mm = exp_avg[e], vv = exp_avg_sq[e];
-            // the clamp on read passes the gradient only inside [-1, 1] (inclusive)
-            const float g = (traw[c] >= -1.0f && traw[c] <= 1.0f) ? gt[c] : 0.0f;
-            adam_update(pp, mm, vv, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
-            tex[e] = pp;
-            exp_avg[e] = mm;
-            exp_avg_sq[e] = vv;
-        }
-    }
-    // per-block partials: loss, dL/dP
+    float gPs[3] = {0.f, 0.f, 0.f};
+    if (all_colocated(cam, light, n_views))
+        fused_pixels<LOSS, OPT_LIGHT, true, NPIX>(px, tex, exp_avg, exp_avg_sq, tgt, npix, n_views, cam, light, P,
+                                                   K, sc, loss, gPs);
+    else
+        fused_pixels<LOSS, OPT_LIGHT, false, NPIX>(px, tex, exp_avg, exp_avg_sq, tgt, npix, n_views, cam, light,
+                                                    P, K, sc, loss, gPs);
+    // per-block partials: loss, dL/dP (plain stores; the finalize kernel that
+    // follows on the same stream reads them)
     const float bl = block_sum(loss, lds);
     float bp[3] = {0.f, 0.f, 0.f};
     if (OPT_LIGHT) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bp[c] = block_sum(a.gP[c], lds);
+        for (int c = 0; c < 3; ++c) bp[c] = block_sum(gPs[c], lds);
     }
-    if (threadIdx.x == 0) {
-        float4 *dst = reinterpret_cast<float4 *>(partials) + blockIdx.x;
-        *dst = make_float4(bl, bp[0], bp[1], bp[2]);
-        __threadfence();
-        const unsigned done = atomicAdd(&st->blocks_done, 1u);
-        am_last = (done == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    // last block: deterministic reduction of all partials
+    if (threadIdx.x == 0) reinterpret_cast<float4 *>(partials)[blockIdx.x] = make_float4(bl, bp[0], bp[1], bp[2]);
+}
+
+// One block: sums the per-block partials of a fused step in a fixed order
+// (bit-reproducible), records the loss, takes the light-power Adam step when
+// it is optimised (one param group with the maps, src/svbrdf.py:50) and
+// advances the step state for the next launch.  Running as a separate launch
+// on the same stream is what makes the partials visible: a cross-block
+// hand-off inside the step kernel would need an agent-scope release per
+// block, which writes back the whole L2 of the XCD on gfx950.
+template <bool OPT_LIGHT>
+__global__ __launch_bounds__(kBlock) void finalize_step_kernel(
+    const float *__restrict__ partials, unsigned n_partials, double count, float *__restrict__ lpow, double lr,
+    double beta1, double beta2, double eps, svbrdf_step_state *__restrict__ st, float *__restrict__ loss_history,
+    int history_len) {
+    __shared__ double ldsd[kBlock / 64];
     const float4 *parts = reinterpret_cast<const float4 *>(partials);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
+    for (unsigned b = threadIdx.x; b < n_partials; b += kBlock) {
         const float4 q = parts[b];
         acc[0] += q.x; acc[1] += q.y; acc[2] += q.z; acc[3] += q.w;
     }
-    double tot[4];
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
     tot[0] = block_sum_d(acc[0], ldsd);
     if (OPT_LIGHT) {
         for (int c = 1; c < 4; ++c) tot[c] = block_sum_d(acc[c], ldsd);
     }
-    if (threadIdx.x == 0) {
-        const double step = (double)st->step + 1.0;
-        const float L = (float)(tot[0] / ((double)n_views * 3.0 * (double)npix));
-        st->loss = L;
-        if (loss_history && (double)history_len >= step) loss_history[(int)step - 1] = L;
-        if (OPT_LIGHT) {
-            // Adam on light_pow with the same scalars as the maps (one param group)
-            for (int c = 0; c < 3; ++c) {
-                float pp = lpow[c], mm = st->light_exp_avg[c], vv = st->light_exp_avg_sq[c];
-                const float g = (float)tot[1 + c];
-                adam_update(pp, mm, vv, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
-                lpow[c] = pp;
-                st->light_exp_avg[c] = mm;
-                st->light_exp_avg_sq[c] = vv;
-                st->light_grad[c] = g;
-            }
+    if (threadIdx.x != 0) return;
+    const double step = (double)st->step + 1.0;
+    const float L = (float)(tot[0] / count);
+    st->loss = L;
+    if (loss_history && (double)history_len >= step) loss_history[(int)step - 1] = L;
+    if (OPT_LIGHT) {
+        const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+        for (int c = 0; c < 3; ++c) {
+            float pp = lpow[c], mm = st->light_exp_avg[c], vv = st->light_exp_avg_sq[c];
+            const float g = (float)tot[1 + c];
+            adam_update(pp, mm, vv, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
+            lpow[c] = pp;
+            st->light_exp_avg[c] = mm;
+            st->light_exp_avg_sq[c] = vv;
+            st->light_grad[c] = g;
         }
-        st->step = (float)step;
-        st->bias_correction1 = 1.0 - pow(beta1, step + 1.0);
-        st->bias_correction2 = 1.0 - pow(beta2, step + 1.0);
-        st->n_partials = gridDim.x;
-        st->blocks_done = 0u;
     }
+    st->step = (float)step;
+    set_bias_corrections(st, beta1, beta2, step + 1.0);
 }
 
 }  // namespace
@@ -405,7 +547,7 @@ int svbrdf_render(const float *textures, int res, int n_views, float size, const
     int rc = check_common("svbrdf_render", textures, res, n_views, camera_pos, light_pos, light_pow);
     if (rc) return rc;
     if (!images) return fail(SVBRDF_EINVAL, "svbrdf_render: null images");
-    hipLaunchKernelGGL(render_kernel, dim3(n_blocks(res)), dim3(kBlock), 0, (hipStream_t)stream, textures,
+    hipLaunchKernelGGL(render_kernel<1>, dim3(n_blocks(res)), dim3(kBlock), 0, (hipStream_t)stream, textures,
                        res, n_views, size, camera_pos, light_pos, light_pow, images);
     return check_launch("svbrdf_render");
 }
@@ -487,14 +629,14 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
         return fail(SVBRDF_ESPACE, "svbrdf_fused_step: workspace of %zu bytes < %zu needed", workspace_bytes, need);
     if (((uintptr_t)workspace & 15u) || ((uintptr_t)state & 15u))
         return fail(SVBRDF_EINVAL, "svbrdf_fused_step: workspace and state must be 16-byte aligned");
-    const unsigned nb = n_blocks(res);
+    const unsigned nb = n_blocks(res, SVBRDF_FUSED_NPIX);
     hipStream_t s = (hipStream_t)stream;
     svbrdf_step_state *st = (svbrdf_step_state *)state;
     float *ws = (float *)workspace;
 #define SVBRDF_LAUNCH_STEP(L, O)                                                                             \
     hipLaunchKernelGGL((fused_step_kernel<L, O>), dim3(nb), dim3(kBlock), 0, s, textures, exp_avg, exp_avg_sq, \
                        targets_prepared, res, n_views, size, camera_pos, light_pos, light_pow, lr, beta1,      \
-                       beta2, eps, st, loss_history, history_len, ws)
+                       beta2, eps, st, ws)
     if (loss_mode == SVBRDF_LOSS_LOG_L2) {
         if (optim_light) SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, true);
         else SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, false);
@@ -502,6 +644,14 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
         if (optim_light) SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_L2, true);
         else SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_L2, false);
     }
+    if ((rc = check_launch("svbrdf_fused_step"))) return rc;
+    const double count = (double)n_views * 3.0 * (double)res * (double)res;
+    if (optim_light)
+        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kBlock), 0, s, ws, nb, count, light_pow, lr,
+                           beta1, beta2, eps, st, loss_history, history_len);
+    else
+        hipLaunchKernelGGL(finalize_step_kernel<false>, dim3(1), dim3(kBlock), 0, s, ws, nb, count, light_pow, lr,
+                           beta1, beta2, eps, st, loss_history, history_len);
 #undef SVBRDF_LAUNCH_STEP
     return check_launch("svbrdf_fused_step");
 }

@@ -7,7 +7,9 @@ Forward images (FP32):
   the GPU's max relative error against the FP64 render is at most twice the
   reference FP32's own (+1e-6).  On well-conditioned pixels sigma ~ 0 and the
   bound is the plain 1e-5 relative / 1e-6 absolute; additionally at least
-  99.9 % of all elements must meet that plain bound against the FP64 render.
+  99.9 % of all elements must meet that plain bound against the FP64 render
+  (or, where the reference FP32 itself meets it for fewer, no fewer than the
+  reference FP32 does, less 0.1 %).
 
 Gradients (FP32), per texture channel:
   ||g_gpu - g64|| <= max(1e-4 ||g64||, 2 ||g_ref32 - g64||): within 1e-4
@@ -28,7 +30,11 @@ def check_forward(out, ref32, ref64, sigma, label=""):
                            f"{(err - bound).max():.3e} at {np.unravel_index((err - bound).argmax(), err.shape)}")
     frac_plain = float(np.mean(err <= plain))
     frac64 = float(np.mean(np.abs(out - ref64) <= 1e-6 + 1e-5 * np.abs(ref64)))
-    assert frac64 >= 0.999, f"{label}: only {frac64:.5f} of elements within 1e-5 rel / 1e-6 abs of FP64"
+    # where FP32 itself cannot reach 1e-5 of FP64 (e.g. the fp32 constant 1 - 1e-6
+    # of src/microfacet.py:67 at grazing normals) the reference FP32 sets the bar
+    frac64_ref = float(np.mean(np.abs(ref32 - ref64) <= 1e-6 + 1e-5 * np.abs(ref64)))
+    assert frac64 >= min(0.999, frac64_ref - 1e-3), (
+        f"{label}: only {frac64:.5f} of elements within 1e-5 rel / 1e-6 abs of FP64 (reference FP32: {frac64_ref:.5f})")
     e_gpu = float((np.abs(out - ref64) / ref64).max())
     e_ref = float((np.abs(ref32 - ref64) / ref64).max())
     assert e_gpu <= 2 * e_ref + 1e-6, f"{label}: max rel err vs FP64 {e_gpu:.3e} > 2 x reference FP32's {e_ref:.3e}"

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 def test_queries(lib, pkg):
     assert lib.svbrdf_abi_version() == 1
     # the host-side view of svbrdf_step_state matches the C struct
-    assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 80
+    assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
     assert lib.svbrdf_max_views() >= 16
     # one float4 partial per 256-pixel block
