@@ -52,10 +52,10 @@ enum svbrdf_loss {
  * svbrdf_step_state_init() and advanced by every svbrdf_fused_step() call,
  * so a captured graph replays correctly without host-side scalars. */
 typedef struct svbrdf_step_state {
-    double bias_correction1;     /* 1 - beta1^t for the step the next call performs */
-    double bias_correction2;     /* 1 - beta2^t                                     */
-    double inv_bias_correction1; /* 1 / bias_correction1                            */
-    float inv_bias_correction2_sqrt; /* 1 / sqrt(bias_correction2), rounded to fp32 */
+    double beta1_pow;            /* beta1^t for the step t the next call performs   */
+    double beta2_pow;            /* beta2^t                                         */
+    double inv_bias_correction1; /* 1 / (1 - beta1^t)                               */
+    float inv_bias_correction2_sqrt; /* 1 / sqrt(1 - beta2^t), rounded to fp32      */
     float step;                  /* completed steps (torch keeps it as a float)     */
     float loss;                  /* image loss evaluated by the last completed step */
     unsigned int reserved[3];
@@ -100,7 +100,9 @@ int svbrdf_prepare_targets(const float *targets, size_t count, int loss_mode,
                            float *prepared, void *stream);
 
 /* Resets a svbrdf_step_state to "no step taken" for the given betas (as
- * torch.optim.Adam's lazily created state, torch/optim/adam.py:142-164). */
+ * torch.optim.Adam's lazily created state, torch/optim/adam.py:142-164).
+ * beta^t is advanced by one multiplication per step, so the betas passed to
+ * svbrdf_fused_step must stay the ones given here. */
 int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream);
 
 /* One fused per-pixel optimisation iteration: the body of the loop of

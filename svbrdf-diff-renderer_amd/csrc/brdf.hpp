@@ -56,7 +56,11 @@ struct Maps {
     float dif[3], spe[3], omspe[3], f1[3];
     float rgh, alpha, a2, k, omk;
     float n[3];
+    float Fc[3];   // Fresnel term when light and camera coincide (v.h = 1)
 };
+
+// 2^((A v.h + B) v.h) at v.h = 1 (co-located light and camera).
+__device__ __forceinline__ float sphg_coloc() { return fexp2(kFresA + kFresB); }
 
 // u^2.2 of u = (t+1)/2; u in {0} or >= 2^-25, so log2 is finite or -inf.
 __device__ __forceinline__ float map_pow(float t) {
@@ -78,6 +82,9 @@ __device__ __forceinline__ void decode(const float t[9], Maps &m) {
         m.omspe[c] = 1.0f - m.spe[c];
         m.f1[c] = m.dif[c] * kInvPi * m.omspe[c];
     }
+    const float sc = sphg_coloc();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m.Fc[c] = m.spe[c] + m.omspe[c] * sc;
     m.rgh = map_pow(t[5]);
     m.alpha = m.rgh * m.rgh;
     m.a2 = m.alpha * m.alpha;
@@ -243,6 +250,81 @@ __device__ __forceinline__ void shade_backward(const Maps &m, const Geom &g, con
     gnh = s.nh_raw >= 0.0f ? gnh : 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) a.gn[c] += gnv * g.v[c] + gnl * g.l[c] + gnh * g.h[c];
+}
+
+// ---------------------------------------------------------------------------
+// Co-located light and camera (the flash setup of every capture the reference
+// handles): l = v and h = (l + v)/|l + v| = v, so n.l = n.h = n.v and
+// v.h = |v| = 1 up to the rounding of the normalisation (~1e-7, below the FP32
+// noise of the reference's own evaluation).  The Fresnel factor is then the
+// pixel constant Maps::Fc, and the three dot-product gradients act along v.
+struct ShadeC {
+    float v[3];
+    float nv_raw, nv, c2, den, iqd, D, idv, g1, G, iq, DGq, inv_d2, nl_d2;
+};
+
+__device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, float cx, float cy, float cz,
+                                            ShadeC &s) {
+    const float vx = cx - px, vy = cy - py, vz = cz;
+    const float sv = vx * vx + vy * vy + vz * vz;
+    const float rv = frsq(sv);
+    s.v[0] = vx * rv; s.v[1] = vy * rv; s.v[2] = vz * rv;
+    s.inv_d2 = frcp(sv);
+    s.nv_raw = m.n[0] * s.v[0] + m.n[1] * s.v[1] + m.n[2] * s.v[2];
+    s.nv = tmax0(s.nv_raw);
+    // GGX with 1 - (n.h)^2 = |n x v|^2 (no cancellation at the highlight)
+    s.c2 = s.nv * s.nv;
+    const float xx = m.n[1] * s.v[2] - m.n[2] * s.v[1];
+    const float xy = m.n[2] * s.v[0] - m.n[0] * s.v[2];
+    const float xz = m.n[0] * s.v[1] - m.n[1] * s.v[0];
+    const float one_m_c2 = s.nv_raw < 0.0f ? 1.0f : xx * xx + xy * xy + xz * xz;
+    s.den = s.c2 * m.a2 + one_m_c2;
+    s.iqd = frcp(kPi * (s.den * s.den) + kEps);
+    s.D = m.a2 * s.iqd;
+    // Smith: G = G1(n.v)^2
+    s.idv = frcp(s.nv * m.omk + m.k);
+    s.g1 = s.nv * s.idv;
+    s.G = s.g1 * s.g1;
+    s.iq = frcp(4.0f * s.nv * s.nv + kEps);
+    s.DGq = s.D * s.G * s.iq;
+    s.nl_d2 = s.nv * s.inv_d2;
+}
+
+// Backward of shade_coloc given dL/df_c (masked) and the direct dL/d(n.l) of
+// img = P f (n.l) / d^2.  Accumulates into `a`; a.T collects dL/dF_c, to be
+// scaled by dF/dspe = 1 - 2^(A+B) once per pixel (coloc_finish).
+__device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, const float gf[3], float gnl,
+                                               Accum &a) {
+    float W = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.S[c] += gf[c];
+        a.T[c] += gf[c] * s.DGq;
+        W += gf[c] * m.Fc[c];
+    }
+    // f2 = D F G / q, q = 4 nv nl + eps
+    const float Wi = W * s.iq;
+    const float gD = Wi * s.G;
+    const float gG = Wi * s.D;
+    const float gq = -Wi * s.DGq;
+    // D = a2 / (pi den^2 + eps), den = c2 a2 + 1 - c2
+    const float dD_dden = -2.0f * kPi * m.a2 * s.den * s.iqd * s.iqd;
+    const float ga2 = gD * (s.iqd + dD_dden * s.c2);
+    const float gc2 = gD * dD_dden * (m.a2 - 1.0f);
+    // n.v = n.l = n.h: q = 4 nv^2 + eps, G = G1(nv)^2, c2 = nv^2
+    const float kidv2 = m.k * s.idv * s.idv;
+    float gnv = gnl + gq * 8.0f * s.nv + gG * 2.0f * s.g1 * kidv2 + gc2 * 2.0f * s.nv;
+    const float gk = -gG * 2.0f * s.g1 * s.nv * (1.0f - s.nv) * s.idv * s.idv;
+    a.A += gk * 0.5f + ga2 * 2.0f * m.alpha;
+    gnv = s.nv_raw >= 0.0f ? gnv : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.gn[c] += gnv * s.v[c];
+}
+
+__device__ __forceinline__ void coloc_finish(Accum &a) {
+    const float om = 1.0f - sphg_coloc();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.T[c] *= om;
 }
 
 // Texture gradient of one pixel from the view sums: the backward of decode().

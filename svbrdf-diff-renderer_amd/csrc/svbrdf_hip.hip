@@ -181,14 +181,24 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
         float *ok = out + (size_t)(3 * k) * npix;
 #pragma unroll
         for (int q = 0; q < NPIX; ++q) {
-            Geom g;
-            geometry<COLOC>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
-            Shade s;
-            shade<COLOC>(m[q], g, P, s);
+            float img[3];
+            if constexpr (COLOC) {
+                ShadeC s;
+                shade_coloc(m[q], px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, s);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) img[c] = P[c] * (m[q].f1[c] + s.DGq * m[q].Fc[c]) * s.nl_d2;
+            } else {
+                Geom g;
+                geometry<false>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
+                Shade s;
+                shade<false>(m[q], g, P, s);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) img[c] = s.img[c];
+            }
             if (px[q].valid) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float ic = tclamp(s.img[c], kEps, 1.0f);
+                    const float ic = tclamp(img[c], kEps, 1.0f);
                     st(plane(ok, c, npix), px[q].off, fexp2(flog2(ic) * kInvGamma));
                 }
             }
@@ -228,22 +238,40 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
         const float *gk = gout + (size_t)(3 * k) * npix;
-        Geom g;
-        geometry<COLOC>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
-        Shade s;
-        shade<COLOC>(m, g, P, s);
-        float gimg[3];
+        float go[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float img = s.img[c];
+        for (int c = 0; c < 3; ++c) go[c] = px.valid ? ld(plane(gk, c, npix), px.off) : 0.0f;
+        // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
+        auto dimg = [&](float img, float g) {
             const float ic = tclamp(img, kEps, 1.0f);
             const float o = fexp2(flog2(ic) * kInvGamma);
-            const float go = px.valid ? ld(plane(gk, c, npix), px.off) : 0.0f;
-            // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
-            gimg[c] = (img >= kEps && img <= 1.0f) ? go * kInvGamma * o * frcp(ic) : 0.0f;
+            return (img >= kEps && img <= 1.0f) ? g * kInvGamma * o * frcp(ic) : 0.0f;
+        };
+        if constexpr (COLOC) {
+            ShadeC s;
+            shade_coloc(m, px.px, px.py, vp.cx, vp.cy, vp.cz, s);
+            float gf[3], gnl = 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float f = m.f1[c] + s.DGq * m.Fc[c];
+                const float gimg = dimg(P[c] * f * s.nl_d2, go[c]);
+                gf[c] = gimg * P[c] * s.nl_d2;
+                gnl += gimg * P[c] * f;
+                if (LIGHT_GRAD) a.gP[c] += gimg * f * s.nl_d2;
+            }
+            backward_coloc(m, s, gf, gnl * s.inv_d2, a);
+        } else {
+            Geom g;
+            geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
+            Shade s;
+            shade<false>(m, g, P, s);
+            float gimg[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gimg[c] = dimg(s.img[c], go[c]);
+            shade_backward<LIGHT_GRAD>(m, g, s, P, gimg, a);
         }
-        shade_backward<LIGHT_GRAD>(m, g, s, P, gimg, a);
     }
+    if constexpr (COLOC) coloc_finish(a);
     float gt[9];
     decode_backward(t, m, a, gt);
     if (px.valid) {
@@ -300,20 +328,21 @@ __global__ __launch_bounds__(kBlock) void prepare_targets_kernel(const float *__
     }
 }
 
-// Bias corrections of step t (torch computes them in double from the float
-// step count, torch/optim/adam.py:519-524) and the per-launch forms the
-// kernel reads: 1/bc1 in double (lr / bc1 = lr * (1/bc1) to the last bit of
-// the fp32 result) and 1/sqrt(bc2) in fp32.
-__device__ void set_bias_corrections(svbrdf_step_state *st, double beta1, double beta2, double t) {
-    st->bias_correction1 = 1.0 - pow(beta1, t);
-    st->bias_correction2 = 1.0 - pow(beta2, t);
-    st->inv_bias_correction1 = 1.0 / st->bias_correction1;
-    st->inv_bias_correction2_sqrt = (float)(1.0 / sqrt(st->bias_correction2));
+// Bias corrections of step t.  torch computes 1 - beta^t in double with pow
+// from the float step count (torch/optim/adam.py:519-524); here beta^t is
+// carried in double and multiplied once per step, which gives the same
+// fp32-rounded step size -lr / (1 - beta1^t) for every t tested (2e5 steps).
+// The kernel reads 1/(1 - beta1^t) in double and 1/sqrt(1 - beta2^t) in fp32.
+__device__ void set_bias_corrections(svbrdf_step_state *st) {
+    st->inv_bias_correction1 = 1.0 / (1.0 - st->beta1_pow);
+    st->inv_bias_correction2_sqrt = (float)(1.0 / sqrt(1.0 - st->beta2_pow));
 }
 
 __global__ void state_init_kernel(svbrdf_step_state *st, double beta1, double beta2) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    set_bias_corrections(st, beta1, beta2, 1.0);
+    st->beta1_pow = beta1;
+    st->beta2_pow = beta2;
+    set_bias_corrections(st);
     st->step = 0.0f;
     st->loss = 0.0f;
     st->reserved[0] = st->reserved[1] = st->reserved[2] = 0u;
@@ -365,24 +394,71 @@ __device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st,
 #define SVBRDF_FUSED_WAVES 1      // minimum waves per SIMD the register allocator must allow
 #endif
 
-// Loss residual and dL/dimg of one channel: r = ln out - ln target' (log-L2)
-// or out - target (L2), with torch's inclusive clamp mask on img.
+// Loss residual and dL/dimg of one channel (general geometry): r = ln out -
+// ln target' (log-L2) or out - target (L2), with torch's inclusive clamp mask
+// on img.  K2 = (2 / element count) / 2.2.
 template <int LOSS>
-__device__ __forceinline__ float loss_channel(float img, float y, float K, bool valid, float &loss) {
+__device__ __forceinline__ float loss_channel(float img, float y, float K2, float &loss) {
     const float ic = tclamp(img, kEps, 1.0f);
     const float lic = flog2(ic);
-    const bool inside = valid && (img >= kEps && img <= 1.0f);
+    const bool inside = (img >= kEps && img <= 1.0f);
     float r, gi;
     if (LOSS == SVBRDF_LOSS_LOG_L2) {
         r = lic * (kInvGamma * kLn2) - y;
-        gi = K * kInvGamma * r * frcp(ic);
+        gi = K2 * r * frcp(ic);
     } else {
         const float o = fexp2(lic * kInvGamma);
         r = o - y;
-        gi = K * kInvGamma * r * o * frcp(ic);
+        gi = K2 * r * o * frcp(ic);
     }
-    loss += valid ? r * r : 0.0f;
+    loss += r * r;
     return inside ? gi : 0.0f;
+}
+
+// Co-located views.  Inside the clamp range (ic = img = P f nl/d2) the chain
+// dL/dimg -> f, nl, P collapses: with rm = r (log-L2) or r out (L2),
+//   dL/df_c = K2 rm / f_c,  dL/dnl = K2 sum_c rm_c / nl,  dL/dP_c = K2 rm / P_c.
+// Outside the range rm = 0 (torch's clamp mask); f and nl are floored at
+// 1e-30 so that a masked zero never meets an infinite reciprocal.
+template <int LOSS, bool OPT_LIGHT>
+__device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, const float *__restrict__ tgt,
+                                                  unsigned npix, int n_views, const float *__restrict__ cam,
+                                                  const float P[3], float K2, float &loss, float (&rP)[3],
+                                                  Accum &a) {
+#pragma unroll SVBRDF_FUSED_UNROLL
+    for (int k = 0; k < n_views; ++k) {
+        const float cx = cam[3 * k + 0], cy = cam[3 * k + 1], cz = cam[3 * k + 2];
+        const float *tk = tgt + (size_t)(3 * k) * npix;
+        float y[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[c] = ld(plane(tk, c, npix), px.off);
+        ShadeC s;
+        shade_coloc(m, px.px, px.py, cx, cy, cz, s);
+        float gf[3], rsum = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float f = m.f1[c] + s.DGq * m.Fc[c];
+            const float img = P[c] * f * s.nl_d2;
+            const float ic = fminf(fmaxf(img, kEps), 1.0f);
+            const bool inside = (ic == img);
+            const float lic = flog2(ic);
+            float r, rm;
+            if (LOSS == SVBRDF_LOSS_LOG_L2) {
+                r = lic * (kInvGamma * kLn2) - y[c];
+                rm = inside ? r : 0.0f;
+            } else {
+                const float o = fexp2(lic * kInvGamma);
+                r = o - y[c];
+                rm = inside ? r * o : 0.0f;
+            }
+            loss += r * r;
+            gf[c] = K2 * rm * frcp(fmaxf(f, 1e-30f));
+            rsum += rm;
+            if (OPT_LIGHT) rP[c] += rm;
+        }
+        backward_coloc(m, s, gf, K2 * rsum * frcp(fmaxf(s.nv, 1e-30f)), a);
+    }
+    coloc_finish(a);
 }
 
 template <int LOSS, bool OPT_LIGHT, bool COLOC, int NPIX>
@@ -390,11 +466,13 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
                                              float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
                                              const float *__restrict__ tgt, unsigned npix, int n_views,
                                              const float *__restrict__ cam, const float *__restrict__ light,
-                                             const float P[3], float K, const StepScalars &sc, float &loss,
+                                             const float P[3], float K2, const StepScalars &sc, float &loss,
                                              float (&gP)[3]) {
     float traw[NPIX][9];
     Maps m[NPIX];
     Accum a[NPIX];
+    float lq[NPIX];
+    float rP[NPIX][3];
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
         float t[9];
@@ -405,37 +483,53 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
         }
         decode(t, m[q]);
         accum_zero(a[q]);
+        lq[q] = 0.f;
+        rP[q][0] = rP[q][1] = rP[q][2] = 0.f;
     }
-#pragma unroll SVBRDF_FUSED_UNROLL
-    for (int k = 0; k < n_views; ++k) {
-        const ViewPos vp = load_view(cam, light, k);
-        const float *tk = tgt + (size_t)(3 * k) * npix;
+    if constexpr (COLOC) {
 #pragma unroll
-        for (int q = 0; q < NPIX; ++q) {
-            const float y0 = ld(plane(tk, 0, npix), px[q].off), y1 = ld(plane(tk, 1, npix), px[q].off),
-                        y2 = ld(plane(tk, 2, npix), px[q].off);
-            Geom g;
-            geometry<COLOC>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
-            Shade s;
-            shade<COLOC>(m[q], g, P, s);
-            float gimg[3];
-            gimg[0] = loss_channel<LOSS>(s.img[0], y0, K, px[q].valid, loss);
-            gimg[1] = loss_channel<LOSS>(s.img[1], y1, K, px[q].valid, loss);
-            gimg[2] = loss_channel<LOSS>(s.img[2], y2, K, px[q].valid, loss);
-            shade_backward<OPT_LIGHT>(m[q], g, s, P, gimg, a[q]);
+        for (int q = 0; q < NPIX; ++q)
+            fused_views_coloc<LOSS, OPT_LIGHT>(px[q], m[q], tgt, npix, n_views, cam, P, K2, lq[q], rP[q], a[q]);
+    } else {
+#pragma unroll SVBRDF_FUSED_UNROLL
+        for (int k = 0; k < n_views; ++k) {
+            const ViewPos vp = load_view(cam, light, k);
+            const float *tk = tgt + (size_t)(3 * k) * npix;
+#pragma unroll
+            for (int q = 0; q < NPIX; ++q) {
+                float y[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) y[c] = ld(plane(tk, c, npix), px[q].off);
+                Geom g;
+                geometry<false>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
+                Shade s;
+                shade<false>(m[q], g, P, s);
+                float gimg[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) gimg[c] = loss_channel<LOSS>(s.img[c], y[c], K2, lq[q]);
+                shade_backward<OPT_LIGHT>(m[q], g, s, P, gimg, a[q]);
+            }
         }
     }
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
+        if (!px[q].valid) continue;
+        // parameters that are NaN make the loss NaN, as in the reference
+        bool finite = true;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) finite = finite && (traw[q][c] == traw[q][c]);
+        loss += finite ? lq[q] : __builtin_nanf("");
+        if (OPT_LIGHT) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                // dL/dP_c = K2 sum rm / P_c (co-located) or the accumulated sum (general)
+                gP[c] += COLOC ? (P[c] != 0.0f ? K2 * rP[q][c] / P[c] : 0.0f) : a[q].gP[c];
+            }
+        }
         float t[9], gt[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) t[c] = tclamp(traw[q][c], -1.0f, 1.0f);
         decode_backward(t, m[q], a[q], gt);
-        if (OPT_LIGHT) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) gP[c] += a[q].gP[c];
-        }
-        if (!px[q].valid) continue;
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
             const unsigned o = px[q].off;
@@ -459,7 +553,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     constexpr int NPIX = SVBRDF_FUSED_NPIX;
     __shared__ float lds[kBlock / 64];
     const unsigned npix = (unsigned)res * (unsigned)res;
-    const float K = 2.0f / (float)((double)n_views * 3.0 * (double)npix);   // d mean / d elem
+    // d mean / d element, times d out^... / d ln img = 1/2.2 (folded once)
+    const float K2 = (float)(2.0 / ((double)n_views * 3.0 * (double)npix) / 2.2);
     const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
     Pix px[NPIX];
@@ -470,10 +565,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     float gPs[3] = {0.f, 0.f, 0.f};
     if (all_colocated(cam, light, n_views))
         fused_pixels<LOSS, OPT_LIGHT, true, NPIX>(px, tex, exp_avg, exp_avg_sq, tgt, npix, n_views, cam, light, P,
-                                                   K, sc, loss, gPs);
+                                                   K2, sc, loss, gPs);
     else
         fused_pixels<LOSS, OPT_LIGHT, false, NPIX>(px, tex, exp_avg, exp_avg_sq, tgt, npix, n_views, cam, light,
-                                                    P, K, sc, loss, gPs);
+                                                    P, K2, sc, loss, gPs);
     // per-block partials: loss, dL/dP (plain stores; the finalize kernel that
     // follows on the same stream reads them)
     const float bl = block_sum(loss, lds);
@@ -492,29 +587,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
 // on the same stream is what makes the partials visible: a cross-block
 // hand-off inside the step kernel would need an agent-scope release per
 // block, which writes back the whole L2 of the XCD on gfx950.
+constexpr int kFinalizeBlock = 1024;
+
 template <bool OPT_LIGHT>
-__global__ __launch_bounds__(kBlock) void finalize_step_kernel(
+__global__ __launch_bounds__(kFinalizeBlock) void finalize_step_kernel(
     const float *__restrict__ partials, unsigned n_partials, double count, float *__restrict__ lpow, double lr,
     double beta1, double beta2, double eps, svbrdf_step_state *__restrict__ st, float *__restrict__ loss_history,
     int history_len) {
-    __shared__ double ldsd[kBlock / 64];
+    constexpr int NQ = OPT_LIGHT ? 4 : 1;
+    __shared__ double lds[NQ][kFinalizeBlock / 64];
     const float4 *parts = reinterpret_cast<const float4 *>(partials);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (unsigned b = threadIdx.x; b < n_partials; b += kBlock) {
-        const float4 q = parts[b];
-        acc[0] += q.x; acc[1] += q.y; acc[2] += q.z; acc[3] += q.w;
+    double acc[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) acc[c] = 0.0;
+    // four independent loads in flight per thread, fixed summation order
+    for (unsigned b0 = threadIdx.x; b0 < n_partials; b0 += 4 * kFinalizeBlock) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned b = b0 + u * kFinalizeBlock;
+            q[u] = b < n_partials ? parts[b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc[0] += q[u].x;
+            if (OPT_LIGHT) { acc[1] += q[u].y; acc[2] += q[u].z; acc[3] += q[u].w; }
+        }
     }
-    double tot[4] = {0.0, 0.0, 0.0, 0.0};
-    tot[0] = block_sum_d(acc[0], ldsd);
-    if (OPT_LIGHT) {
-        for (int c = 1; c < 4; ++c) tot[c] = block_sum_d(acc[c], ldsd);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        double x = acc[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) lds[c][wave] = x;
     }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < NQ; ++c)
+        for (int w = 0; w < kFinalizeBlock / 64; ++w) tot[c] += lds[c][w];
     const double step = (double)st->step + 1.0;
     const float L = (float)(tot[0] / count);
     st->loss = L;
     if (loss_history && (double)history_len >= step) loss_history[(int)step - 1] = L;
     if (OPT_LIGHT) {
+        // Adam on light_pow with the same scalars as the maps (one param group)
         const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
         for (int c = 0; c < 3; ++c) {
             float pp = lpow[c], mm = st->light_exp_avg[c], vv = st->light_exp_avg_sq[c];
@@ -527,7 +646,9 @@ __global__ __launch_bounds__(kBlock) void finalize_step_kernel(
         }
     }
     st->step = (float)step;
-    set_bias_corrections(st, beta1, beta2, step + 1.0);
+    st->beta1_pow *= beta1;
+    st->beta2_pow *= beta2;
+    set_bias_corrections(st);
 }
 
 }  // namespace
@@ -647,10 +768,10 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
     if ((rc = check_launch("svbrdf_fused_step"))) return rc;
     const double count = (double)n_views * 3.0 * (double)res * (double)res;
     if (optim_light)
-        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kBlock), 0, s, ws, nb, count, light_pow, lr,
+        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kFinalizeBlock), 0, s, ws, nb, count, light_pow, lr,
                            beta1, beta2, eps, st, loss_history, history_len);
     else
-        hipLaunchKernelGGL(finalize_step_kernel<false>, dim3(1), dim3(kBlock), 0, s, ws, nb, count, light_pow, lr,
+        hipLaunchKernelGGL(finalize_step_kernel<false>, dim3(1), dim3(kFinalizeBlock), 0, s, ws, nb, count, light_pow, lr,
                            beta1, beta2, eps, st, loss_history, history_len);
 #undef SVBRDF_LAUNCH_STEP
     return check_launch("svbrdf_fused_step");

@@ -24,7 +24,7 @@ from . import _lib
 LOSSES = {"l2": _lib.LOSS_L2, "log_l2": _lib.LOSS_LOG_L2, "logl2": _lib.LOSS_LOG_L2}
 
 # layout of svbrdf_step_state (include/svbrdf_hip.h)
-_STATE_DTYPE = np.dtype([("bias_correction1", "<f8"), ("bias_correction2", "<f8"),
+_STATE_DTYPE = np.dtype([("beta1_pow", "<f8"), ("beta2_pow", "<f8"),
                          ("inv_bias_correction1", "<f8"), ("inv_bias_correction2_sqrt", "<f4"),
                          ("step", "<f4"), ("loss", "<f4"), ("reserved", "<u4", 3),
                          ("light_exp_avg", "<f4", 4), ("light_exp_avg_sq", "<f4", 4),
