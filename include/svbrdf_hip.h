@@ -131,6 +131,64 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq,
                       int history_len, void *workspace, size_t workspace_bytes,
                       void *stream);
 
+/* ------------------------------------------------------------------------
+ * Batched step: many independent materials (uneven resolution and view count)
+ * in one persistent launch.  The reference runs one material per process
+ * (src/scripts.py:67-97); this is the same per-pixel iteration applied to
+ * every material of a batch at once.  Each material is one svbrdf_material
+ * (device pointers, host-side sizes); the host plans tiles once, copies the
+ * table and the two tile arrays to device memory, and then calls
+ * svbrdf_batch_step once per iteration.  Blocks of a persistent grid take
+ * tiles from an atomic ticket counter in cost order (views x pixels, largest
+ * first); per-material losses are reduced in a fixed order (bit-reproducible).
+ * Light-power optimisation is not available in batched mode.
+ */
+typedef struct svbrdf_material {
+    float *textures;                /* [9,R,R], updated in place                   */
+    float *exp_avg;                 /* [9,R,R], zero before the first step          */
+    float *exp_avg_sq;              /* [9,R,R], zero before the first step          */
+    const float *targets_prepared;  /* [V,3,R,R] from svbrdf_prepare_targets        */
+    const float *camera_pos;        /* [V,3] cm                                     */
+    const float *light_pos;         /* [V,3] cm                                     */
+    const float *light_pow;         /* [3]                                          */
+    int res;                        /* R                                            */
+    int n_views;                    /* V                                            */
+    float size;                     /* plane width, cm                              */
+    unsigned int tile_begin;        /* set by svbrdf_batch_plan                     */
+    unsigned int n_tiles;           /* set by svbrdf_batch_plan                     */
+    unsigned int reserved;
+} svbrdf_material;
+
+/* Number of tiles of `tile_pixels` pixels (a multiple of 256) the batch splits
+ * into; 0 on invalid input. */
+size_t svbrdf_batch_tile_count(const svbrdf_material *materials, int n_materials, int tile_pixels);
+
+/* Host-side plan: fills tile_begin / n_tiles of every material (host array)
+ * and writes the execution order (tile ids, costliest first) and the tile ->
+ * material map, each of svbrdf_batch_tile_count() entries. */
+int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixels, unsigned int *order,
+                      unsigned int *tile_material, size_t n_tiles);
+
+/* Workspace: per-tile partial sums plus the ticket counter; must be zeroed
+ * once before the first step (each step leaves the counter at zero). */
+size_t svbrdf_batch_workspace_bytes(size_t n_tiles);
+
+/* Number of blocks that fit on the device at once for a kernel variant
+ * (pixels_per_thread in {1, 2}, views_per_trip in {1, 3}): the grid size of a
+ * persistent launch is blocks_per_cu x compute units. */
+int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int views_per_trip);
+
+/* One optimisation iteration of every material of the batch (device copies of
+ * the table and tile arrays).  losses[m] receives material m's loss;
+ * loss_history (optional) is [history_len][n_materials].  state is shared by
+ * the batch (all materials step together). */
+int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const unsigned int *order,
+                      const unsigned int *tile_material, size_t n_tiles, int tile_pixels,
+                      int pixels_per_thread, int views_per_trip, int grid_blocks, int loss_mode,
+                      double lr, double beta1, double beta2, double eps, void *state, float *losses,
+                      float *loss_history, int history_len, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,8 @@ Drop-in for the hot path of tflsguoyu/svbrdf-diff-renderer:
 * :class:`SvbrdfOptim` -- src/svbrdf.py:15-83 (one fused kernel per iteration)
 * :class:`FusedStep` -- the device-resident optimisation problem behind it
   (graph capture, exact resume)
+* :class:`BatchedStep` -- many materials of uneven size in one persistent
+  launch per iteration, with an autotuned launch shape
 
 The directory name is not a Python identifier; it is imported under the
 module name ``svbrdf_amd`` (see ``load_package`` in __graft_entry__.py).
@@ -13,8 +15,9 @@ All compute runs in libsvbrdf_hip.so (gfx950); nothing falls back to the CPU.
 """
 from . import _lib, synthetic
 from ._lib import LOSS_L2, LOSS_LOG_L2, SvbrdfHipError
+from .batch import BatchedStep
 from .optim import FusedStep, SvbrdfOptim, loss_mode
 from .renderer import Microfacet, RenderFunction
 
-__all__ = ["Microfacet", "RenderFunction", "SvbrdfOptim", "FusedStep", "loss_mode", "synthetic",
+__all__ = ["Microfacet", "RenderFunction", "SvbrdfOptim", "FusedStep", "BatchedStep", "loss_mode", "synthetic",
            "LOSS_L2", "LOSS_LOG_L2", "SvbrdfHipError"]

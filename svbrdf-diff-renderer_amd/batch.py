@@ -1,0 +1,257 @@
+"""Many materials, one persistent launch per iteration.
+
+The reference optimises one material per process (src/scripts.py:67-97).
+``BatchedStep`` runs the same per-pixel iteration (clamp, render, loss,
+adjoint, Adam; src/svbrdf.py:58-71) for a whole batch of materials of
+uneven resolution and view count in one persistent kernel: blocks take
+fixed-size pixel tiles from an atomic ticket counter, costliest tiles
+(views x pixels) first.
+
+The launch shape -- tile size, pixels per thread, views per loop trip and
+blocks per compute unit -- is chosen by :meth:`BatchedStep.autotune`, which
+times each candidate on the batch itself (state saved and restored around
+the trials) and caches the winner per (device, loss, batch shape) in a JSON
+file.
+"""
+import ctypes
+import hashlib
+import json
+import os
+import pathlib
+
+import numpy as np
+import torch
+
+from . import _lib
+from .optim import _STATE_DTYPE, loss_mode
+
+DEFAULT_CONFIG = {"tile_pixels": 4096, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": 0}
+
+
+class Material(ctypes.Structure):
+    """ctypes view of ``svbrdf_material`` (include/svbrdf_hip.h)."""
+    _fields_ = [("textures", ctypes.c_void_p), ("exp_avg", ctypes.c_void_p), ("exp_avg_sq", ctypes.c_void_p),
+                ("targets_prepared", ctypes.c_void_p), ("camera_pos", ctypes.c_void_p),
+                ("light_pos", ctypes.c_void_p), ("light_pow", ctypes.c_void_p), ("res", ctypes.c_int),
+                ("n_views", ctypes.c_int), ("size", ctypes.c_float), ("tile_begin", ctypes.c_uint),
+                ("n_tiles", ctypes.c_uint), ("reserved", ctypes.c_uint)]
+
+
+def _cache_path():
+    root = os.environ.get("SVBRDF_AMD_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "svbrdf_amd")
+    return pathlib.Path(root) / "autotune.json"
+
+
+def _compute_units(device):
+    return torch.cuda.get_device_properties(device).multi_processor_count
+
+
+class BatchedStep:
+    """A batch of independent per-pixel optimisation problems.
+
+    problems: sequence of (renderer, textures, targets) with renderer a
+    Microfacet (geometry, light power, size), textures a contiguous fp32
+    CUDA tensor of 9*R*R values (updated in place) and targets [V,3,R,R].
+    All materials share lr, betas, eps and the loss, and step together.
+    """
+
+    def __init__(self, problems, lr, loss="log_l2", betas=(0.9, 0.999), eps=1e-8, history=0, config=None):
+        if not problems:
+            raise ValueError("empty batch")
+        self.device = problems[0][0].device
+        self.lr, self.beta1, self.beta2, self.eps = float(lr), float(betas[0]), float(betas[1]), float(eps)
+        self.mode = loss_mode(loss)
+        s = _lib.stream_handle(self.device)
+        self.renderers, self.textures, self.exp_avg, self.exp_avg_sq, self.targets = [], [], [], [], []
+        for r, tex, tgt in problems:
+            if tex.device != self.device or tex.dtype != torch.float32 or not tex.is_contiguous():
+                raise ValueError("textures must be contiguous float32 tensors on one CUDA device")
+            if tex.numel() != 9 * r.res * r.res or tuple(tgt.shape) != (r.n_of_imgs, 3, r.res, r.res):
+                raise ValueError(f"material of res {r.res} / {r.n_of_imgs} views: shape mismatch")
+            tg = tgt.detach().to(self.device, torch.float32).contiguous()
+            prepared = torch.empty_like(tg)
+            _lib.call("svbrdf_prepare_targets", _lib.ptr(tg), tg.numel(), self.mode, _lib.ptr(prepared), s)
+            self.renderers.append(r)
+            self.textures.append(tex)
+            self.exp_avg.append(torch.zeros_like(tex))
+            self.exp_avg_sq.append(torch.zeros_like(tex))
+            self.targets.append(prepared)
+        self.light_pow = [r.light_pow.detach().to(self.device, torch.float32).reshape(3).contiguous()
+                          for r in self.renderers]
+        self.n = len(problems)
+        nstate = _lib.query("svbrdf_step_state_bytes")
+        self.state = torch.zeros((nstate + 15) // 16 * 4, dtype=torch.float32, device=self.device)
+        _lib.call("svbrdf_step_state_init", _lib.ptr(self.state), self.beta1, self.beta2, s)
+        self.losses_dev = torch.zeros(self.n, dtype=torch.float32, device=self.device)
+        self.history = torch.zeros((max(int(history), 0), self.n), dtype=torch.float32, device=self.device)
+        self.steps_launched = 0
+        self.graph = None
+        self.cu = _compute_units(self.device)
+        self.configure(dict(DEFAULT_CONFIG, **(config or {})))
+
+    # ------------------------------------------------------------------ plan
+    def signature(self):
+        shape = sorted((r.res, r.n_of_imgs) for r in self.renderers)
+        h = hashlib.sha1(json.dumps(shape).encode()).hexdigest()[:16]
+        return f"{torch.cuda.get_device_name(self.device)}|loss{self.mode}|{h}"
+
+    def configure(self, cfg):
+        """Plans tiles for ``cfg`` and builds the device tables."""
+        cfg = dict(cfg)
+        npix_t, unroll = int(cfg["pixels_per_thread"]), int(cfg["views_per_trip"])
+        tile = int(cfg["tile_pixels"])
+        if tile % (256 * npix_t):
+            raise ValueError(f"tile_pixels {tile} is not a multiple of {256 * npix_t}")
+        max_bpc = _lib.query("svbrdf_batch_max_blocks_per_cu", self.mode, npix_t, unroll)
+        if max_bpc < 1:
+            raise RuntimeError("svbrdf_batch_max_blocks_per_cu failed: " + _lib.load().svbrdf_last_error().decode())
+        bpc = int(cfg.get("blocks_per_cu") or 0)
+        bpc = max_bpc if bpc <= 0 else min(bpc, max_bpc)
+        cfg["blocks_per_cu"] = bpc
+        table = (Material * self.n)()
+        for i, r in enumerate(self.renderers):
+            table[i] = Material(self.textures[i].data_ptr(), self.exp_avg[i].data_ptr(),
+                                self.exp_avg_sq[i].data_ptr(), self.targets[i].data_ptr(),
+                                r.camera_pos.data_ptr(), r.light_pos.data_ptr(), self.light_pow[i].data_ptr(),
+                                r.res, r.n_of_imgs, r.size, 0, 0, 0)
+        n_tiles = _lib.query("svbrdf_batch_tile_count", table, self.n, tile)
+        if n_tiles == 0:
+            raise ValueError("invalid batch for tile planning")
+        order = (ctypes.c_uint * n_tiles)()
+        tile_mat = (ctypes.c_uint * n_tiles)()
+        _lib.call("svbrdf_batch_plan", table, self.n, tile, order, tile_mat, n_tiles)
+        self.table_host = table
+        self.table = torch.frombuffer(bytearray(bytes(table)), dtype=torch.uint8).to(self.device)
+        self.order = torch.frombuffer(bytearray(bytes(order)), dtype=torch.int32).to(self.device)
+        self.tile_mat = torch.frombuffer(bytearray(bytes(tile_mat)), dtype=torch.int32).to(self.device)
+        nws = _lib.query("svbrdf_batch_workspace_bytes", n_tiles)
+        self.workspace = torch.zeros(nws // 4 + 64, dtype=torch.float32, device=self.device)
+        off = (-self.workspace.data_ptr()) % 256
+        self.ws_ptr = self.workspace.data_ptr() + off
+        self.ws_bytes = self.workspace.numel() * 4 - off
+        self.n_tiles = n_tiles
+        self.config = cfg
+        self.grid = bpc * self.cu
+        self.graph = None
+        return cfg
+
+    # ----------------------------------------------------------------- steps
+    def _launch(self, stream):
+        c = self.config
+        hist = self.history if self.history.numel() else None
+        _lib.call("svbrdf_batch_step", _lib.ptr(self.table), self.n, _lib.ptr(self.order), _lib.ptr(self.tile_mat),
+                  self.n_tiles, c["tile_pixels"], c["pixels_per_thread"], c["views_per_trip"], self.grid,
+                  self.mode, self.lr, self.beta1, self.beta2, self.eps, _lib.ptr(self.state),
+                  _lib.ptr(self.losses_dev), _lib.ptr(hist), self.history.shape[0], ctypes.c_void_p(self.ws_ptr),
+                  self.ws_bytes, stream)
+
+    def step(self):
+        self._launch(_lib.stream_handle(self.device))
+        self.steps_launched += 1
+
+    def capture(self):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            g.capture_begin()
+            self._launch(ctypes.c_void_p(s.cuda_stream))
+            g.capture_end()
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        self.steps_launched += 1
+
+    def losses(self):
+        """Per-material losses of the last step (synchronises)."""
+        return self.losses_dev.cpu().numpy()
+
+    def loss_history(self):
+        n = min(self.steps_launched, self.history.shape[0])
+        return self.history[:n].cpu().numpy()
+
+    def read_state(self):
+        raw = self.state.detach().cpu().numpy().view(np.uint8)[:_STATE_DTYPE.itemsize]
+        return raw.view(_STATE_DTYPE)[0]
+
+    # -------------------------------------------------------------- autotune
+    def _snapshot(self):
+        return ([t.clone() for t in self.textures], [m.clone() for m in self.exp_avg],
+                [v.clone() for v in self.exp_avg_sq], self.state.clone(), self.losses_dev.clone(),
+                self.history.clone(), self.steps_launched)
+
+    def _restore(self, snap):
+        with torch.no_grad():
+            for dst, src in zip(self.textures, snap[0]):
+                dst.copy_(src)
+            for dst, src in zip(self.exp_avg, snap[1]):
+                dst.copy_(src)
+            for dst, src in zip(self.exp_avg_sq, snap[2]):
+                dst.copy_(src)
+            self.state.copy_(snap[3])
+            self.losses_dev.copy_(snap[4])
+            self.history.copy_(snap[5])
+        self.steps_launched = snap[6]
+
+    @staticmethod
+    def candidates():
+        out = []
+        for npix_t in (1, 2):
+            for unroll in (1, 3):
+                for tile in (1024, 4096, 16384):
+                    for bpc in (0, 2):
+                        out.append({"tile_pixels": tile, "pixels_per_thread": npix_t, "views_per_trip": unroll,
+                                    "blocks_per_cu": bpc})
+        return out
+
+    def autotune(self, candidates=None, iters=3, use_cache=True, verbose=False):
+        """Times each candidate launch shape on this batch and keeps the
+        fastest; the optimisation state is restored afterwards.  Returns
+        (config, {config key: ms per step})."""
+        key = self.signature()
+        path = _cache_path()
+        if use_cache and candidates is None:
+            try:
+                cached = json.loads(path.read_text()).get(key)
+            except (OSError, ValueError):
+                cached = None
+            if cached:
+                return self.configure(cached), {}
+        snap = self._snapshot()
+        timings = {}
+        best = None
+        for cand in candidates or self.candidates():
+            cfg = self.configure(cand)
+            k = json.dumps(cfg, sort_keys=True)
+            if k in timings:
+                continue
+            for _ in range(2):
+                self.step()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize(self.device)
+            e0.record()
+            for _ in range(iters):
+                self.step()
+            e1.record()
+            torch.cuda.synchronize(self.device)
+            ms = e0.elapsed_time(e1) / iters
+            timings[k] = ms
+            if verbose:
+                print(f"autotune {k}: {ms:.3f} ms", flush=True)
+            if best is None or ms < best[0]:
+                best = (ms, cfg)
+            self._restore(snap)
+        self._restore(snap)
+        cfg = self.configure(best[1])
+        if use_cache:
+            try:
+                path.parent.mkdir(parents=True, exist_ok=True)
+                data = json.loads(path.read_text()) if path.exists() else {}
+                data[key] = cfg
+                path.write_text(json.dumps(data, indent=1, sort_keys=True))
+            except (OSError, ValueError):
+                pass
+        return cfg, timings

@@ -1,0 +1,80 @@
+"""Render adjoint composed with a feature-space loss.
+
+The reference's second caller of the renderer (MaterialGAN,
+src/materialgan.py:136-152) renders, passes the images through a VGG19
+feature extractor (src/descriptor.py:7-79: pretrained VGG19, max-pool
+replaced by average-pool) and back-propagates a feature loss through the
+renderer.  No pretrained weights are available offline, so this module
+builds a VGG-19-SHAPED stack with seeded He-normal weights: conv1_1 ...
+conv5_1 (3x3 convolutions, ReLU, 2x2 average pooling between stages).
+The loss is the L2 distance of Gram matrices at relu1_1, relu2_1,
+relu3_1, relu4_1, relu5_1 plus a pixel L2 term.  The renderer's adjoint
+kernel receives dL/dimages from torch's convolution backward through
+:class:`svbrdf_amd.RenderFunction`.
+"""
+import torch
+
+# (out_channels) per conv layer, 'P' = 2x2 average pool; up to conv5_1
+VGG19_TO_CONV5_1 = [64, 64, "P", 128, 128, "P", 256, 256, 256, 256, "P", 512, 512, 512, 512, "P", 512]
+STYLE_LAYERS = (0, 2, 4, 8, 12)   # indices of conv1_1, conv2_1, conv3_1, conv4_1, conv5_1 among the convs
+
+
+class VGGShapedFeatures(torch.nn.Module):
+    """Random-init (He-normal, seeded) VGG-19-shaped conv stack up to conv5_1."""
+
+    def __init__(self, seed=4, device="cuda", dtype=torch.float32):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        self.convs = torch.nn.ModuleList()
+        self.plan = []
+        cin = 3
+        for item in VGG19_TO_CONV5_1:
+            if item == "P":
+                self.plan.append("P")
+                continue
+            conv = torch.nn.Conv2d(cin, item, 3, padding=1)
+            with torch.no_grad():
+                std = (2.0 / (cin * 9)) ** 0.5
+                conv.weight.copy_(torch.randn(conv.weight.shape, generator=g) * std)
+                conv.bias.zero_()
+            self.convs.append(conv)
+            self.plan.append(len(self.convs) - 1)
+            cin = item
+        self.to(device=device, dtype=dtype)
+        for p in self.parameters():
+            p.requires_grad_(False)
+
+    def forward(self, x):
+        feats = []
+        for step in self.plan:
+            if step == "P":
+                x = torch.nn.functional.avg_pool2d(x, 2)
+            else:
+                x = torch.relu(self.convs[step](x))
+                if step in STYLE_LAYERS:
+                    feats.append(x)
+        return feats
+
+
+def gram(f):
+    n, c, h, w = f.shape
+    f = f.reshape(n, c, h * w)
+    return torch.bmm(f, f.transpose(1, 2).contiguous()) / (c * h * w)
+
+
+class FeatureLoss:
+    """L2 of Gram matrices on the 5 style layers + pixel L2 (weights w_gram, w_pixel)."""
+
+    def __init__(self, net, targets, w_gram=1.0, w_pixel=1.0):
+        self.net = net
+        self.targets = targets.detach()
+        with torch.no_grad():
+            self.target_grams = [gram(f) for f in net(self.targets)]
+        self.w_gram, self.w_pixel = w_gram, w_pixel
+
+    def __call__(self, images):
+        feats = self.net(images)
+        loss = self.w_pixel * torch.nn.functional.mse_loss(images, self.targets)
+        for f, gt in zip(feats, self.target_grams):
+            loss = loss + self.w_gram * torch.nn.functional.mse_loss(gram(f), gt)
+        return loss

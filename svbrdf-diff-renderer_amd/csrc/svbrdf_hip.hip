@@ -13,12 +13,14 @@
 // i, so every plane access of a wavefront is one contiguous 256-byte run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "brdf.hpp"
 #include "../../include/svbrdf_hip.h"
@@ -420,12 +422,12 @@ __device__ __forceinline__ float loss_channel(float img, float y, float K2, floa
 //   dL/df_c = K2 rm / f_c,  dL/dnl = K2 sum_c rm_c / nl,  dL/dP_c = K2 rm / P_c.
 // Outside the range rm = 0 (torch's clamp mask); f and nl are floored at
 // 1e-30 so that a masked zero never meets an infinite reciprocal.
-template <int LOSS, bool OPT_LIGHT>
+template <int LOSS, bool OPT_LIGHT, int UNROLL>
 __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, const float *__restrict__ tgt,
                                                   unsigned npix, int n_views, const float *__restrict__ cam,
                                                   const float P[3], float K2, float &loss, float (&rP)[3],
                                                   Accum &a) {
-#pragma unroll SVBRDF_FUSED_UNROLL
+#pragma unroll UNROLL
     for (int k = 0; k < n_views; ++k) {
         const float cx = cam[3 * k + 0], cy = cam[3 * k + 1], cz = cam[3 * k + 2];
         const float *tk = tgt + (size_t)(3 * k) * npix;
@@ -461,7 +463,7 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
     coloc_finish(a);
 }
 
-template <int LOSS, bool OPT_LIGHT, bool COLOC, int NPIX>
+template <int LOSS, bool OPT_LIGHT, bool COLOC, int NPIX, int UNROLL>
 __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__restrict__ tex,
                                              float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
                                              const float *__restrict__ tgt, unsigned npix, int n_views,
@@ -489,9 +491,10 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
     if constexpr (COLOC) {
 #pragma unroll
         for (int q = 0; q < NPIX; ++q)
-            fused_views_coloc<LOSS, OPT_LIGHT>(px[q], m[q], tgt, npix, n_views, cam, P, K2, lq[q], rP[q], a[q]);
+            fused_views_coloc<LOSS, OPT_LIGHT, UNROLL>(px[q], m[q], tgt, npix, n_views, cam, P, K2, lq[q], rP[q],
+                                                       a[q]);
     } else {
-#pragma unroll SVBRDF_FUSED_UNROLL
+#pragma unroll UNROLL
         for (int k = 0; k < n_views; ++k) {
             const ViewPos vp = load_view(cam, light, k);
             const float *tk = tgt + (size_t)(3 * k) * npix;
@@ -564,11 +567,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     float loss = 0.f;
     float gPs[3] = {0.f, 0.f, 0.f};
     if (all_colocated(cam, light, n_views))
-        fused_pixels<LOSS, OPT_LIGHT, true, NPIX>(px, tex, exp_avg, exp_avg_sq, tgt, npix, n_views, cam, light, P,
-                                                   K2, sc, loss, gPs);
+        fused_pixels<LOSS, OPT_LIGHT, true, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, npix,
+                                                                        n_views, cam, light, P, K2, sc, loss, gPs);
     else
-        fused_pixels<LOSS, OPT_LIGHT, false, NPIX>(px, tex, exp_avg, exp_avg_sq, tgt, npix, n_views, cam, light,
-                                                    P, K2, sc, loss, gPs);
+        fused_pixels<LOSS, OPT_LIGHT, false, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, npix,
+                                                                         n_views, cam, light, P, K2, sc, loss, gPs);
     // per-block partials: loss, dL/dP (plain stores; the finalize kernel that
     // follows on the same stream reads them)
     const float bl = block_sum(loss, lds);
@@ -611,7 +614,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void finalize_step_kernel(
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             acc[0] += q[u].x;
-            if (OPT_LIGHT) { acc[1] += q[u].y; acc[2] += q[u].z; acc[3] += q[u].w; }
+            if constexpr (OPT_LIGHT) { acc[1] += q[u].y; acc[2] += q[u].z; acc[3] += q[u].w; }
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -649,6 +652,101 @@ __global__ __launch_bounds__(kFinalizeBlock) void finalize_step_kernel(
     st->beta1_pow *= beta1;
     st->beta2_pow *= beta2;
     set_bias_corrections(st);
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent step.  A grid sized to the device pulls tiles from an
+// atomic ticket counter; the counter is the only state blocks share, so no
+// fence is needed, and every block leaves once its ticket passes n_tiles.
+template <int LOSS, int NPIX, int UNROLL>
+__global__ __launch_bounds__(kBlock) void batch_step_kernel(
+    const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
+    const unsigned *__restrict__ tile_mat, unsigned n_tiles, int tile_pixels, unsigned *__restrict__ queue,
+    float4 *__restrict__ partials, double lr, double beta1, double beta2, double eps,
+    const svbrdf_step_state *__restrict__ st) {
+    __shared__ float lds[kBlock / 64];
+    __shared__ unsigned s_slot;
+    const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+    for (;;) {
+        if (threadIdx.x == 0) s_slot = atomicAdd(queue, 1u);
+        __syncthreads();
+        const unsigned slot = s_slot;
+        __syncthreads();
+        if (slot >= n_tiles) return;
+        const unsigned id = order[slot];
+        const svbrdf_material mt = mats[tile_mat[id]];
+        const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
+        const unsigned first = (id - mt.tile_begin) * (unsigned)tile_pixels;
+        const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
+        const float P[3] = {mt.light_pow[0], mt.light_pow[1], mt.light_pow[2]};
+        const bool coloc = all_colocated(mt.camera_pos, mt.light_pos, mt.n_views);
+        float loss = 0.f;
+        float gPs[3] = {0.f, 0.f, 0.f};
+        for (unsigned base = 0; base < (unsigned)tile_pixels && first + base < npix; base += kBlock * NPIX) {
+            Pix px[NPIX];
+#pragma unroll
+            for (int q = 0; q < NPIX; ++q)
+                px[q] = pixel_at(first + base + q * kBlock + threadIdx.x, npix, mt.res, mt.size);
+            if (coloc)
+                fused_pixels<LOSS, false, true, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
+                                                              mt.targets_prepared, npix, mt.n_views, mt.camera_pos,
+                                                              mt.light_pos, P, K2, sc, loss, gPs);
+            else
+                fused_pixels<LOSS, false, false, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
+                                                               mt.targets_prepared, npix, mt.n_views, mt.camera_pos,
+                                                               mt.light_pos, P, K2, sc, loss, gPs);
+        }
+        const float bl = block_sum(loss, lds);
+        if (threadIdx.x == 0) partials[id] = make_float4(bl, 0.f, 0.f, 0.f);
+    }
+}
+
+// Block m sums material m's tile partials in tile order (fixed, so the loss is
+// bit-reproducible) and records its loss; reads the step count only.
+__global__ __launch_bounds__(kBlock) void batch_reduce_kernel(const svbrdf_material *__restrict__ mats,
+                                                              const float4 *__restrict__ partials,
+                                                              const svbrdf_step_state *__restrict__ st,
+                                                              float *__restrict__ losses,
+                                                              float *__restrict__ loss_history, int history_len,
+                                                              int n_materials) {
+    __shared__ double ldsd[kBlock / 64];
+    const svbrdf_material mt = mats[blockIdx.x];
+    double acc = 0.0;
+    for (unsigned b = threadIdx.x; b < mt.n_tiles; b += kBlock) acc += (double)partials[mt.tile_begin + b].x;
+    const double tot = block_sum_d(acc, ldsd);
+    if (threadIdx.x != 0) return;
+    const double count = (double)mt.n_views * 3.0 * (double)mt.res * (double)mt.res;
+    const float L = (float)(tot / count);
+    losses[blockIdx.x] = L;
+    const int t = (int)st->step;   // completed steps before this one
+    if (loss_history && t < history_len) loss_history[(size_t)t * n_materials + blockIdx.x] = L;
+}
+
+// Advances the shared step state and re-arms the ticket counter.
+__global__ void batch_advance_kernel(svbrdf_step_state *__restrict__ st, unsigned *__restrict__ queue, double beta1,
+                                     double beta2) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->step = st->step + 1.0f;
+    st->beta1_pow *= beta1;
+    st->beta2_pow *= beta2;
+    set_bias_corrections(st);
+    *queue = 0u;
+}
+
+size_t batch_partials_bytes(size_t n_tiles) { return (n_tiles * sizeof(float4) + 255) / 256 * 256; }
+
+template <int L>
+const void *batch_kernel_ptr(int npix, int unroll) {
+    if (npix == 1) return unroll == 3 ? (const void *)batch_step_kernel<L, 1, 3> : (const void *)batch_step_kernel<L, 1, 1>;
+    return unroll == 3 ? (const void *)batch_step_kernel<L, 2, 3> : (const void *)batch_step_kernel<L, 2, 1>;
+}
+
+int check_variant(const char *what, int loss_mode, int npix, int unroll) {
+    if (loss_mode != SVBRDF_LOSS_L2 && loss_mode != SVBRDF_LOSS_LOG_L2)
+        return fail(SVBRDF_EINVAL, "%s: unknown loss_mode %d", what, loss_mode);
+    if (npix != 1 && npix != 2) return fail(SVBRDF_EINVAL, "%s: pixels_per_thread must be 1 or 2", what);
+    if (unroll != 1 && unroll != 3) return fail(SVBRDF_EINVAL, "%s: views_per_trip must be 1 or 3", what);
+    return SVBRDF_OK;
 }
 
 }  // namespace
@@ -775,6 +873,121 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
                            beta1, beta2, eps, st, loss_history, history_len);
 #undef SVBRDF_LAUNCH_STEP
     return check_launch("svbrdf_fused_step");
+}
+
+// ---------------------------------------------------------------------------
+// Batched step
+
+size_t svbrdf_batch_tile_count(const svbrdf_material *materials, int n_materials, int tile_pixels) {
+    if (!materials || n_materials < 1 || tile_pixels < kBlock || tile_pixels % kBlock) return 0;
+    size_t n = 0;
+    for (int i = 0; i < n_materials; ++i) {
+        const int r = materials[i].res;
+        if (r < 1 || r > kMaxRes) return 0;
+        n += ((size_t)r * r + tile_pixels - 1) / tile_pixels;
+    }
+    return n;
+}
+
+int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixels, unsigned int *order,
+                      unsigned int *tile_material, size_t n_tiles) {
+    if (!materials || !order || !tile_material || n_materials < 1)
+        return fail(SVBRDF_EINVAL, "svbrdf_batch_plan: null pointer or empty batch");
+    if (tile_pixels < kBlock || tile_pixels % kBlock)
+        return fail(SVBRDF_EINVAL, "svbrdf_batch_plan: tile_pixels=%d must be a positive multiple of %d",
+                    tile_pixels, kBlock);
+    for (int i = 0; i < n_materials; ++i) {
+        const svbrdf_material &m = materials[i];
+        if (!m.textures || !m.exp_avg || !m.exp_avg_sq || !m.targets_prepared || !m.camera_pos || !m.light_pos ||
+            !m.light_pow)
+            return fail(SVBRDF_EINVAL, "svbrdf_batch_plan: material %d has a null pointer", i);
+        if (m.res < 1 || m.res > kMaxRes || m.n_views < 1 || m.n_views > kMaxViews)
+            return fail(SVBRDF_ESHAPE, "svbrdf_batch_plan: material %d has res=%d n_views=%d", i, m.res, m.n_views);
+    }
+    const size_t need = svbrdf_batch_tile_count(materials, n_materials, tile_pixels);
+    if (need != n_tiles)
+        return fail(SVBRDF_ESPACE, "svbrdf_batch_plan: tile arrays hold %zu entries, the batch has %zu tiles",
+                    n_tiles, need);
+    if (n_tiles > 0xffffffffull) return fail(SVBRDF_ESHAPE, "svbrdf_batch_plan: too many tiles");
+    std::vector<double> cost(n_tiles);
+    unsigned t = 0;
+    for (int i = 0; i < n_materials; ++i) {
+        svbrdf_material &m = materials[i];
+        const size_t npix = (size_t)m.res * m.res;
+        m.tile_begin = t;
+        m.n_tiles = (unsigned)((npix + tile_pixels - 1) / tile_pixels);
+        for (unsigned k = 0; k < m.n_tiles; ++k, ++t) {
+            tile_material[t] = (unsigned)i;
+            const size_t px = std::min<size_t>(tile_pixels, npix - (size_t)k * tile_pixels);
+            cost[t] = (double)px * (double)m.n_views;
+        }
+    }
+    for (size_t k = 0; k < n_tiles; ++k) order[k] = (unsigned)k;
+    std::stable_sort(order, order + n_tiles, [&](unsigned a, unsigned b) { return cost[a] > cost[b]; });
+    return SVBRDF_OK;
+}
+
+size_t svbrdf_batch_workspace_bytes(size_t n_tiles) { return batch_partials_bytes(n_tiles) + 256; }
+
+int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int views_per_trip) {
+    if (check_variant("svbrdf_batch_max_blocks_per_cu", loss_mode, pixels_per_thread, views_per_trip)) return 0;
+    const void *k = loss_mode == SVBRDF_LOSS_LOG_L2 ? batch_kernel_ptr<SVBRDF_LOSS_LOG_L2>(pixels_per_thread, views_per_trip)
+                                                    : batch_kernel_ptr<SVBRDF_LOSS_L2>(pixels_per_thread, views_per_trip);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const unsigned int *order,
+                      const unsigned int *tile_material, size_t n_tiles, int tile_pixels, int pixels_per_thread,
+                      int views_per_trip, int grid_blocks, int loss_mode, double lr, double beta1, double beta2,
+                      double eps, void *state, float *losses, float *loss_history, int history_len,
+                      void *workspace, size_t workspace_bytes, void *stream) {
+    const char *what = "svbrdf_batch_step";
+    int rc = check_variant(what, loss_mode, pixels_per_thread, views_per_trip);
+    if (rc) return rc;
+    if (!materials || !order || !tile_material || !state || !losses || !workspace || n_materials < 1)
+        return fail(SVBRDF_EINVAL, "%s: null pointer argument or empty batch", what);
+    if (n_tiles == 0 || n_tiles > 0xffffffffull) return fail(SVBRDF_ESHAPE, "%s: n_tiles=%zu", what, n_tiles);
+    if (tile_pixels < kBlock * pixels_per_thread || tile_pixels % (kBlock * pixels_per_thread))
+        return fail(SVBRDF_EINVAL, "%s: tile_pixels=%d must be a multiple of %d", what, tile_pixels,
+                    kBlock * pixels_per_thread);
+    if (grid_blocks < 1 || grid_blocks > (1 << 20)) return fail(SVBRDF_EINVAL, "%s: grid_blocks=%d", what, grid_blocks);
+    if (!(lr >= 0.0) || !(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0) || !(eps >= 0.0))
+        return fail(SVBRDF_EINVAL, "%s: invalid Adam hyper-parameters", what);
+    if (workspace_bytes < svbrdf_batch_workspace_bytes(n_tiles))
+        return fail(SVBRDF_ESPACE, "%s: workspace of %zu bytes < %zu needed", what, workspace_bytes,
+                    svbrdf_batch_workspace_bytes(n_tiles));
+    if (((uintptr_t)workspace & 255u) || ((uintptr_t)state & 15u))
+        return fail(SVBRDF_EINVAL, "%s: workspace must be 256-byte and state 16-byte aligned", what);
+    hipStream_t s = (hipStream_t)stream;
+    float4 *partials = (float4 *)workspace;
+    unsigned *queue = (unsigned *)((char *)workspace + batch_partials_bytes(n_tiles));
+    svbrdf_step_state *st = (svbrdf_step_state *)state;
+#define SVBRDF_LAUNCH_BATCH(L, N, U)                                                                          \
+    hipLaunchKernelGGL((batch_step_kernel<L, N, U>), dim3(grid_blocks), dim3(kBlock), 0, s, materials, order, \
+                       tile_material, (unsigned)n_tiles, tile_pixels, queue, partials, lr, beta1, beta2, eps, st)
+#define SVBRDF_LAUNCH_BATCH_L(L)                                                                              \
+    do {                                                                                                       \
+        if (pixels_per_thread == 1) {                                                                          \
+            if (views_per_trip == 3) SVBRDF_LAUNCH_BATCH(L, 1, 3); else SVBRDF_LAUNCH_BATCH(L, 1, 1);          \
+        } else {                                                                                               \
+            if (views_per_trip == 3) SVBRDF_LAUNCH_BATCH(L, 2, 3); else SVBRDF_LAUNCH_BATCH(L, 2, 1);          \
+        }                                                                                                      \
+    } while (0)
+    if (loss_mode == SVBRDF_LOSS_LOG_L2) SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_LOG_L2);
+    else SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_L2);
+#undef SVBRDF_LAUNCH_BATCH_L
+#undef SVBRDF_LAUNCH_BATCH
+    if ((rc = check_launch(what))) return rc;
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(n_materials), dim3(kBlock), 0, s, materials, partials, st, losses,
+                       loss_history, history_len, n_materials);
+    if ((rc = check_launch(what))) return rc;
+    hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, s, st, queue, beta1, beta2);
+    return check_launch(what);
 }
 
 }  // extern "C"

@@ -1,0 +1,111 @@
+"""Batched persistent step: many materials of uneven size and view count in
+one launch.  Each material's per-pixel arithmetic is the single-material
+kernel's, so its parameters must be bit-identical to running it alone
+through FusedStep, whatever the launch shape; losses are reduced per tile
+instead of per block, so they agree to rounding."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import microfacet_np as M
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+# (res, views, co-located)
+SHAPES = [(64, 9, True), (33, 4, True), (100, 16, True), (16, 1, True), (48, 5, False), (128, 12, True)]
+
+
+def make_problems(pkg, shapes, seed=0):
+    out = []
+    for i, (res, V, coloc) in enumerate(shapes):
+        gt = pkg.synthetic.ground_truth_textures(res, seed=seed + i)
+        init = pkg.synthetic.init_textures(gt, seed=seed + 50 + i)
+        rng = np.random.default_rng(seed + 200 + i)
+        cam, light = pkg.synthetic.flash_grid(jitter=3.0, rng=rng, n_views=V)
+        if not coloc:
+            light = (light + rng.uniform(-2, 2, light.shape)).astype(np.float32)
+        P = pkg.synthetic.light_power()
+        size = pkg.synthetic.PLANE_SIZE_CM
+        target = M.render(gt, res, size, cam, light, P, np.float32)
+        out.append((res, V, size, cam, light, P, init, target))
+    return out
+
+
+def build(pkg, specs, kind, **kw):
+    problems, texs = [], []
+    for res, V, size, cam, light, P, init, target in specs:
+        cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+        r = pkg.Microfacet(res, V, size, cl, DEV)
+        tex = torch.tensor(init, device=DEV)[None].contiguous()
+        problems.append((r, tex, torch.tensor(target, device=DEV)))
+        texs.append(tex)
+    if kind == "batch":
+        return pkg.BatchedStep(problems, 0.01, **kw), texs
+    return [pkg.FusedStep(r, t, y, 0.01, history=8, **kw) for r, t, y in problems], texs
+
+
+@pytest.mark.parametrize("cfg", [
+    {"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": 1},
+    {"tile_pixels": 4096, "pixels_per_thread": 1, "views_per_trip": 3, "blocks_per_cu": 0},
+    {"tile_pixels": 1024, "pixels_per_thread": 2, "views_per_trip": 1, "blocks_per_cu": 2},
+    {"tile_pixels": 16384, "pixels_per_thread": 2, "views_per_trip": 3, "blocks_per_cu": 0},
+])
+@pytest.mark.parametrize("loss", ["log_l2", "l2"])
+def test_batch_matches_single_material_steps(pkg, cfg, loss):
+    specs = make_problems(pkg, SHAPES)
+    batch, btex = build(pkg, specs, "batch", loss=loss, history=8, config=cfg)
+    singles, stex = build(pkg, specs, "single", loss=loss)
+    for _ in range(3):
+        batch.step()
+        for f in singles:
+            f.step()
+    torch.cuda.synchronize()
+    for i, (b, s) in enumerate(zip(btex, stex)):
+        np.testing.assert_array_equal(b.cpu().numpy(), s.cpu().numpy(), err_msg=f"material {i} {SHAPES[i]}")
+        np.testing.assert_array_equal(batch.exp_avg_sq[i].cpu().numpy(), singles[i].exp_avg_sq.cpu().numpy())
+    hist = batch.loss_history()
+    assert hist.shape == (3, len(SHAPES))
+    for i, f in enumerate(singles):
+        np.testing.assert_allclose(hist[:, i], f.losses(), rtol=1e-5)
+    assert batch.read_state()["step"] == 3.0
+
+
+def test_batch_graph_replay_and_determinism(pkg):
+    specs = make_problems(pkg, SHAPES, seed=7)
+    a, ta = build(pkg, specs, "batch", history=4)
+    b, tb = build(pkg, specs, "batch", history=4)
+    for _ in range(3):
+        a.step()
+    b.capture()
+    for _ in range(3):
+        b.replay()
+        _ = (tb[0].sum() * 0).item()
+    torch.cuda.synchronize()
+    for x, y in zip(ta, tb):
+        np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy())
+    np.testing.assert_array_equal(a.loss_history(), b.loss_history())
+
+
+def test_autotune_restores_state_and_picks_a_candidate(pkg, tmp_path, monkeypatch):
+    monkeypatch.setenv("SVBRDF_AMD_CACHE", str(tmp_path))
+    specs = make_problems(pkg, SHAPES[:4], seed=3)
+    b, tex = build(pkg, specs, "batch", history=4)
+    b.step()
+    torch.cuda.synchronize()
+    before = [t.cpu().numpy().copy() for t in tex]
+    st_before = b.read_state().copy()
+    cands = pkg.BatchedStep.candidates()[:6]
+    cfg, timings = b.autotune(candidates=cands, iters=2)
+    assert len(timings) >= 1 and all(t > 0 for t in timings.values())
+    assert cfg["tile_pixels"] in (1024, 4096, 16384)
+    after = [t.cpu().numpy() for t in tex]
+    for x, y in zip(before, after):
+        np.testing.assert_array_equal(x, y)
+    assert b.read_state()["step"] == st_before["step"]
+    # the cache file now holds the winner and a second autotune reads it back
+    cfg2, t2 = b.autotune()
+    assert cfg2 == cfg and t2 == {}
+    b.step()
+    torch.cuda.synchronize()
+    assert b.read_state()["step"] == 2.0

@@ -1,0 +1,92 @@
+"""Host-side pieces that need no GPU: calibration loading, the synthetic
+workloads, the batch planner's tile tables and bench's torch-autograd
+baseline (which must compute the reference's images)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import microfacet_np as M
+from oracle import microfacet_torch as MT
+from tests.conftest import ROOT
+
+
+def test_load_calibration_selects_views_in_idx_order(pkg):
+    from svbrdf_amd.calibration import load_calibration
+    cl, size, idx = load_calibration(ROOT / "tests" / "fixtures" / "calib_grid.json")
+    assert size == 12.5 and idx == [3, 0, 2]
+    np.testing.assert_array_equal(cl[0].numpy(), [[2, -2, 20], [-2, 2, 20], [-2, -2, 20]])
+    np.testing.assert_array_equal(cl[1].numpy()[:, 2], [21, 21, 21])
+    assert cl[2].dtype == torch.float32 and cl[2].tolist() == [900, 1000, 1100]
+
+
+def test_synthetic_maps_follow_the_recipe(pkg):
+    syn = pkg.synthetic
+    t = syn.ground_truth_textures(128, seed=0)
+    assert t.shape == (9, 128, 128) and t.dtype == np.float32
+    u = (t + 1) / 2
+    assert 0.05 <= u[0:3].min() and u[0:3].max() <= 0.9
+    assert 0.02 <= u[6:9].min() and u[6:9].max() <= 0.3
+    assert 0.1 <= u[5].min() and u[5].max() <= 0.9
+    assert (t[3] ** 2 + t[4] ** 2).max() < 1          # unit normals: |(nx, ny)| < 1
+    cam, light = syn.flash_grid()
+    assert cam.shape == (9, 3) and np.array_equal(cam, light)
+    assert cam[0].tolist() == [-8, 8, 30] and cam[8].tolist() == [8, -8, 30]
+    init = syn.init_textures(t, seed=1)
+    assert np.abs(init).max() <= 1 and 0.05 < np.abs(init - t).mean() < 0.1
+    np.testing.assert_array_equal(syn.ground_truth_textures(64, seed=5), syn.ground_truth_textures(64, seed=5))
+
+
+def test_config3_batch_shape():
+    import bench
+    sizes = [256] * 24 + [512] * 24 + [1024] * 16
+    assert sum(r * r for r in sizes) == 24_641_536
+    rng = np.random.default_rng(3)
+    views = rng.integers(4, 17, len(sizes))
+    assert views.min() >= 4 and views.max() <= 16
+    assert bench.algorithmic_bytes_step(1024, 9) == 339_738_624
+
+
+def test_batch_plan_orders_tiles_by_cost(pkg):
+    from svbrdf_amd.batch import Material
+    lib = pkg._lib.load()
+    shapes = [(256, 5), (1024, 16), (13, 4), (512, 9)]
+    mats = (Material * len(shapes))()
+    for i, (r, v) in enumerate(shapes):
+        mats[i] = Material(0x1000, 0x2000, 0x3000, 0x4000, 0x5000, 0x6000, 0x7000, r, v, 17.0, 0, 0, 0)
+    n = lib.svbrdf_batch_tile_count(mats, len(shapes), 4096)
+    assert n == 16 + 256 + 1 + 64
+    order = (ctypes.c_uint * n)()
+    tile_mat = (ctypes.c_uint * n)()
+    assert lib.svbrdf_batch_plan(mats, len(shapes), 4096, order, tile_mat, n) == 0
+    assert [(m.tile_begin, m.n_tiles) for m in mats] == [(0, 16), (16, 256), (272, 1), (273, 64)]
+    order = list(order)
+    assert sorted(order) == list(range(n))
+
+    def cost(t):
+        r, v = shapes[tile_mat[t]]
+        first = (t - mats[tile_mat[t]].tile_begin) * 4096
+        return min(4096, r * r - first) * v
+
+    costs = [cost(t) for t in order]
+    assert costs == sorted(costs, reverse=True)
+    assert lib.svbrdf_batch_plan(mats, len(shapes), 4096, (ctypes.c_uint * 3)(), tile_mat, 3) == -3
+    assert lib.svbrdf_batch_plan(mats, len(shapes), 1000, (ctypes.c_uint * n)(), tile_mat, n) == -1
+
+
+def test_bench_eager_baseline_renders_the_reference_images():
+    import bench
+    rng = np.random.default_rng(0)
+    res, V = 16, 4
+    tex = rng.uniform(-0.9, 0.9, (1, 9, res, res)).astype(np.float32)
+    tex[:, 3:5] *= 0.4
+    cam = np.array([[x, y, 25.0] for x in (-5.0, 5.0) for y in (-5.0, 5.0)], np.float32)
+    P = np.array([1000, 1200, 1400], np.float32)
+    path = bench.EagerReferencePath(res, V, 17.0, cam, cam, P, "cpu")
+    out = path.eval(torch.tensor(tex)).numpy()
+    ref = M.render(tex[0], res, 17.0, cam, cam, P, np.float64)
+    np.testing.assert_allclose(out, ref, rtol=2e-4, atol=1e-6)
+    cl = [torch.tensor(cam), torch.tensor(cam), torch.tensor(P)]
+    ref32 = MT.MicrofacetEager(res, V, 17.0, cl, "cpu").eval(torch.tensor(tex)).numpy()
+    np.testing.assert_allclose(out, ref32, rtol=2e-4, atol=1e-6)
