@@ -667,12 +667,14 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(
     __shared__ float lds[kBlock / 64];
     __shared__ unsigned s_slot;
     const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
-    for (;;) {
-        if (threadIdx.x == 0) s_slot = atomicAdd(queue, 1u);
-        __syncthreads();
-        const unsigned slot = s_slot;
-        __syncthreads();
-        if (slot >= n_tiles) return;
+    // The ticket is read back through readfirstlane so that the loop exit is a
+    // scalar (wave-uniform) branch: with a divergent exit the control-flow
+    // structurizer may keep lanes re-running a tile while lane 0 waits to draw
+    // the next ticket.  The trip count is also bounded by n_tiles + 1.
+    if (threadIdx.x == 0) s_slot = atomicAdd(queue, 1u);
+    __syncthreads();
+    unsigned slot = __builtin_amdgcn_readfirstlane(s_slot);
+    for (unsigned trip = 0; slot < n_tiles && trip <= n_tiles; ++trip) {
         const unsigned id = order[slot];
         const svbrdf_material mt = mats[tile_mat[id]];
         const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
@@ -696,8 +698,15 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(
                                                                mt.targets_prepared, npix, mt.n_views, mt.camera_pos,
                                                                mt.light_pos, P, K2, sc, loss, gPs);
         }
+        // block_sum ends with a barrier, so every thread has read s_slot before
+        // thread 0 overwrites it with the next ticket
         const float bl = block_sum(loss, lds);
-        if (threadIdx.x == 0) partials[id] = make_float4(bl, 0.f, 0.f, 0.f);
+        if (threadIdx.x == 0) {
+            partials[id] = make_float4(bl, 0.f, 0.f, 0.f);
+            s_slot = atomicAdd(queue, 1u);
+        }
+        __syncthreads();
+        slot = __builtin_amdgcn_readfirstlane(s_slot);
     }
 }
 
