@@ -3,14 +3,19 @@
 One step = one fused iteration of the reference loop body (src/svbrdf.py:58-71):
 clamp -> render -> log-space L2 loss -> backward -> Adam, on a 1024x1024
 material seen from 9 co-located flash views, all fp32, as one kernel launch
-replayed from a HIP graph.  Inputs are synthetic (seeded maps, targets
-rendered from the ground truth) and resident in HBM before timing starts.
+per iteration, replayed from HIP graphs of 10 iterations.  Inputs are
+synthetic (seeded maps, targets rendered from the ground truth) and resident
+in HBM before timing starts.
 
 Printed JSON line (one):
-  value             iterations/sec of the fused step (graph replays, HIP events)
+  value             iterations/sec of the fused step over the K timed
+                    iterations (HIP events around the whole region)
   roofline          algorithmic HBM bytes per iteration (324 B/pixel at V=9:
                     params, exp_avg, exp_avg_sq read+written once, targets read
-                    once) / measured kernel time, against 8 TB/s
+                    once) / the kernel's average launch duration (HIP events
+                    around single launches), against 8 TB/s; step_frac is the
+                    same over the whole-job time per iteration; traffic is the
+                    profiler-measured HBM bytes per launch
   torch_gpu_baseline  the reference's eager op sequence + torch.optim.Adam on
                     the same GPU (it/s), and the speedup over it
   cpu_baseline      the same eager restatement on the host CPU cores (oracle/)
@@ -22,7 +27,7 @@ Printed JSON line (one):
 
     python bench.py [--steps K] [--warmup W] [--res 1024] [--views 9]
                     [--no-cpu] [--no-torch] [--no-batch] [--no-compose]
-                    [--cpu-seconds 20]
+                    [--cpu-seconds 20] [--graph-steps 10 | --eager]
 """
 import argparse
 import json
@@ -42,6 +47,13 @@ import __graft_entry__  # noqa: E402
 
 METRIC = "per-pixel optim iters/sec at 1024²×9 views (fused vs torch autograd); HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+
+
+# HBM bytes per fused-step launch read from the profiler (rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, in KiB per dispatch, FETCH_SIZE scaled by 2.0 as a
+# plain device copy of known size calibrates it, WRITE_SIZE by 1.0), for the
+# shapes that were profiled: 110,686 KiB x 2 + 110,848 KiB at 1024^2 x 9.
+MEASURED_TRAFFIC = {(1024, 9): (110686 * 2 + 110848) * 1024}
 
 
 def algorithmic_bytes_step(res, views):
@@ -68,6 +80,21 @@ def make_problem(pkg, res, views, dev, loss="log_l2"):
     tex = torch.tensor(init, device=dev)[None].contiguous()
     fs = pkg.FusedStep(r, tex, target, lr=0.01, loss=loss, history=0)
     return dict(renderer=r, fused=fs, tex=tex, target=target, gt=gt, init=init, cam=cam, light=light, P=P)
+
+
+def kernel_duration_ms(launch, n):
+    """Average duration of one launch: HIP events recorded right before and
+    after each launch on the launching stream (torch's current stream), summed
+    over n launches; the gaps between launches are excluded."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    launch()
+    torch.cuda.synchronize()
+    for a, b in ev:
+        a.record()
+        launch()
+        b.record()
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / n
 
 
 def time_events(fn, n, stream=None):
@@ -321,6 +348,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-material batched leg (config 3)")
     ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (config 4)")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="iterations captured per HIP graph (graph launches carry a fixed cost each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the bench, run 5 plain device copies of MIB MiB (a known-size kernel that "
                          "calibrates the profiler's FETCH_SIZE / WRITE_SIZE counters)")
@@ -336,31 +365,50 @@ def main():
     prob = make_problem(pkg, res, views, dev)
     fs = prob["fused"]
 
-    # fused step: capture one iteration, warm up, time K replays
+    # fused step: K iterations as eager launches, or as replays of a graph of
+    # GRAPH_STEPS iterations (plus one graph for the remainder), so exactly K
+    # iterations run in the timed region
     if args.eager:
-        run = fs.step
+        def run(n):
+            for _ in range(n):
+                fs.step()
     else:
-        fs.capture()
-        run = fs.replay
-    for _ in range(args.warmup):
-        run()
+        per = max(1, min(args.graph_steps, max(args.steps, 1)))
+        graphs = {per: fs.capture(per)}
+
+        def run(n):
+            for _ in range(n // per):
+                graphs[per].replay()
+            rem = n % per
+            if rem:
+                if rem not in graphs:
+                    graphs[rem] = fs.capture(rem)
+                graphs[rem].replay()
+        run(per + args.steps % per)          # captures every graph before the timed region
+    run(args.warmup)
     torch.cuda.synchronize()
-    ms = time_events(run, args.steps)
+    ms = time_events(lambda: run(args.steps), 1) / args.steps
     loss_after = fs.loss()
     value = 1000.0 / ms
+    kernel_ms = kernel_duration_ms(fs.step, 50)
 
     nbytes = algorithmic_bytes_step(res, views)
-    achieved = nbytes / (ms * 1e-3) / 1e9
+    achieved = nbytes / (kernel_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "it/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"fused per-pixel iteration (clamp+render+log-L2+adjoint+Adam) {res}^2 x {views} "
-                               f"views, one kernel per step, HIP-graph replay",
-                   "res": res, "views": views, "loss": "log_l2", "graph": not args.eager},
+                               f"views, one kernel per step, "
+                               + ("eager launches" if args.eager else f"HIP graphs of {args.graph_steps} steps"),
+                   "res": res, "views": views, "loss": "log_l2", "graph": not args.eager,
+                   "graph_steps": None if args.eager else args.graph_steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes": nbytes, "kernel": "fused_step_kernel"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((res, views)),
+                     "algorithmic_bytes": nbytes, "kernel": "fused_step_kernel",
+                     "kernel_ms": round(kernel_ms, 5),
+                     "step_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "loss_after": loss_after,
     }
 

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 1
+#define SVBRDF_ABI_VERSION 2
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -57,8 +57,9 @@ typedef struct svbrdf_step_state {
     double inv_bias_correction1; /* 1 / (1 - beta1^t)                               */
     float inv_bias_correction2_sqrt; /* 1 / sqrt(1 - beta2^t), rounded to fp32      */
     float step;                  /* completed steps (torch keeps it as a float)     */
-    float loss;                  /* image loss evaluated by the last completed step */
-    unsigned int reserved[3];
+    float loss;                  /* image loss of the last recorded step            */
+    unsigned int pending_step;   /* step whose partial sums await reduction, or 0   */
+    unsigned int reserved[2];
     float light_exp_avg[4];      /* Adam state of light_pow (when optimised)        */
     float light_exp_avg_sq[4];
     float light_grad[4];         /* dL/dlight_pow of the last step (when optimised) */
@@ -114,14 +115,23 @@ int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream
  * inclusive-bound gradient mask).  targets_prepared comes from
  * svbrdf_prepare_targets with the same loss_mode.  When optim_light is
  * non-zero light_pow[3] is optimised too (src/svbrdf.py:48-50,58-59) with
- * its Adam state in `state`.  The loss of this step is left in state->loss
- * and, when loss_history is not NULL and the step index t (0-based) is below
- * history_len, also in loss_history[t] (device memory), so a loop can keep
- * every loss without a host synchronisation per iteration (the reference
- * reads loss.item() each epoch, src/svbrdf.py:64).  The workspace holds
- * per-block partial sums (svbrdf_fused_step_workspace_bytes(res) bytes).
- * The call enqueues two kernels: the per-pixel step and a one-block
- * finalize that reduces the partial sums and advances `state`. */
+ * its Adam state in `state`.
+ *
+ * Losses: the loss of step t (1-based) goes to state->loss and, when
+ * loss_history is not NULL and t <= history_len, to loss_history[t - 1]
+ * (device memory), so a loop keeps every loss without a host
+ * synchronisation per iteration (the reference reads loss.item() each
+ * epoch, src/svbrdf.py:64).  Without light optimisation the call is ONE
+ * kernel launch and the loss of step t is reduced by the launch of step
+ * t + 1 or by svbrdf_fused_step_flush(); with optim_light a one-block
+ * finalize follows the step kernel (the next step's light power needs this
+ * step's full dL/dP) and records the loss at once.
+ *
+ * Workspace: svbrdf_fused_step_workspace_bytes(res) bytes, 256-byte aligned,
+ * zero-filled before the first call and then kept with `state` (it holds
+ * arrival counters, which every launch leaves at zero, and the partial sums
+ * of the last step).  One state + workspace pair serves one problem (one
+ * res); calls that share a pair must be stream-ordered. */
 size_t svbrdf_fused_step_workspace_bytes(int res);
 int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq,
                       const float *targets_prepared, int res, int n_views, float size,
@@ -130,6 +140,14 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq,
                       double beta2, double eps, void *state, float *loss_history,
                       int history_len, void *workspace, size_t workspace_bytes,
                       void *stream);
+
+/* Records the loss of the last step taken by svbrdf_fused_step when it is
+ * still pending (no-op otherwise): state->loss and loss_history[t - 1].
+ * Call it before reading losses on the host (no counterpart in the
+ * reference, where the loss is read synchronously each epoch). */
+int svbrdf_fused_step_flush(void *state, int res, int n_views, float *loss_history,
+                            int history_len, const void *workspace,
+                            size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------
  * Batched step: many independent materials (uneven resolution and view count)

@@ -43,6 +43,8 @@ SIGNATURES = {
                                          ctypes.c_double, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp,
                                          ctypes.c_size_t, _vp]),
     # batched step (svbrdf_material tables are passed as pointers)
+    "svbrdf_fused_step_flush": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                               ctypes.c_size_t, _vp]),
     "svbrdf_batch_tile_count": (ctypes.c_size_t, [_vp, ctypes.c_int, ctypes.c_int]),
     "svbrdf_batch_plan": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_size_t]),
     "svbrdf_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),

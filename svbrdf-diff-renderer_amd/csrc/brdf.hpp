@@ -60,6 +60,10 @@ struct Maps {
 };
 
 // 2^((A v.h + B) v.h) at v.h = 1 (co-located light and camera).
+#ifndef SVBRDF_INV_D2_FROM_RSQ
+#define SVBRDF_INV_D2_FROM_RSQ 0   // 1: 1/d^2 of a co-located view as rsq(d^2)^2 (one transcendental less)
+#endif
+
 __device__ __forceinline__ float sphg_coloc() { return fexp2(kFresA + kFresB); }
 
 // u^2.2 of u = (t+1)/2; u in {0} or >= 2^-25, so log2 is finite or -inf.
@@ -269,7 +273,11 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     const float sv = vx * vx + vy * vy + vz * vz;
     const float rv = frsq(sv);
     s.v[0] = vx * rv; s.v[1] = vy * rv; s.v[2] = vz * rv;
+#if SVBRDF_INV_D2_FROM_RSQ
+    s.inv_d2 = rv * rv;
+#else
     s.inv_d2 = frcp(sv);
+#endif
     s.nv_raw = m.n[0] * s.v[0] + m.n[1] * s.v[1] + m.n[2] * s.v[2];
     s.nv = tmax0(s.nv_raw);
     // GGX with 1 - (n.h)^2 = |n x v|^2 (no cancellation at the highlight)

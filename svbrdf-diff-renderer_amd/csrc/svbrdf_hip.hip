@@ -347,7 +347,8 @@ __global__ void state_init_kernel(svbrdf_step_state *st, double beta1, double be
     set_bias_corrections(st);
     st->step = 0.0f;
     st->loss = 0.0f;
-    st->reserved[0] = st->reserved[1] = st->reserved[2] = 0u;
+    st->pending_step = 0u;
+    st->reserved[0] = st->reserved[1] = 0u;
     for (int c = 0; c < 4; ++c) {
         st->light_exp_avg[c] = 0.0f;
         st->light_exp_avg_sq[c] = 0.0f;
@@ -395,6 +396,18 @@ __device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st,
 #ifndef SVBRDF_FUSED_WAVES
 #define SVBRDF_FUSED_WAVES 1      // minimum waves per SIMD the register allocator must allow
 #endif
+#ifndef SVBRDF_MOMENT_PREFETCH
+// When the Adam moments are loaded: 0 = per channel next to its stores (the
+// compiler cannot reorder a load past a store through another descriptor, so
+// the nine round trips serialise), 1 = all 18 after the view loop and before
+// the first store (one round trip; measured 69 vs 76 us per step at
+// 1024^2 x 9), 2 = all 18 before the view loop (18 more VGPRs through the
+// loop: 126, slower).
+#define SVBRDF_MOMENT_PREFETCH 1
+#endif
+#ifndef SVBRDF_TARGET_PREFETCH
+#define SVBRDF_TARGET_PREFETCH 0  // 1: load view k+1's targets while view k is shaded (no gain measured)
+#endif
 
 // Loss residual and dL/dimg of one channel (general geometry): r = ln out -
 // ln target' (log-L2) or out - target (L2), with torch's inclusive clamp mask
@@ -427,13 +440,29 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
                                                   unsigned npix, int n_views, const float *__restrict__ cam,
                                                   const float P[3], float K2, float &loss, float (&rP)[3],
                                                   Accum &a) {
+    float yn[3];
+    if (SVBRDF_TARGET_PREFETCH) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) yn[c] = ld(plane(tgt, c, npix), px.off);
+    }
 #pragma unroll UNROLL
     for (int k = 0; k < n_views; ++k) {
         const float cx = cam[3 * k + 0], cy = cam[3 * k + 1], cz = cam[3 * k + 2];
-        const float *tk = tgt + (size_t)(3 * k) * npix;
         float y[3];
+        if (SVBRDF_TARGET_PREFETCH) {
+            // issue view k+1's loads now (the last view re-reads its own, an L1/L2 hit)
+            const int kn = k + 1 < n_views ? k + 1 : k;
+            const float *tn = tgt + (size_t)(3 * kn) * npix;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) y[c] = ld(plane(tk, c, npix), px.off);
+            for (int c = 0; c < 3; ++c) {
+                y[c] = yn[c];
+                yn[c] = ld(plane(tn, c, npix), px.off);
+            }
+        } else {
+            const float *tk = tgt + (size_t)(3 * k) * npix;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) y[c] = ld(plane(tk, c, npix), px.off);
+        }
         ShadeC s;
         shade_coloc(m, px.px, px.py, cx, cy, cz, s);
         float gf[3], rsum = 0.f;
@@ -471,18 +500,25 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
                                              const float P[3], float K2, const StepScalars &sc, float &loss,
                                              float (&gP)[3]) {
     float traw[NPIX][9];
+    float mm[NPIX][9], vv[NPIX][9];
     Maps m[NPIX];
     Accum a[NPIX];
     float lq[NPIX];
     float rP[NPIX][3];
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) traw[q][c] = ld(plane(tex, c, npix), px[q].off);
+        if (SVBRDF_MOMENT_PREFETCH == 2) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                mm[q][c] = ld(plane(exp_avg, c, npix), px[q].off);
+                vv[q][c] = ld(plane(exp_avg_sq, c, npix), px[q].off);
+            }
+        }
         float t[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            traw[q][c] = ld(plane(tex, c, npix), px[q].off);
-            t[c] = tclamp(traw[q][c], -1.0f, 1.0f);   // textures.clamp(-1, 1), svbrdf.py:60
-        }
+        for (int c = 0; c < 9; ++c) t[c] = tclamp(traw[q][c], -1.0f, 1.0f);   // textures.clamp(-1, 1), svbrdf.py:60
         decode(t, m[q]);
         accum_zero(a[q]);
         lq[q] = 0.f;
@@ -494,15 +530,30 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
             fused_views_coloc<LOSS, OPT_LIGHT, UNROLL>(px[q], m[q], tgt, npix, n_views, cam, P, K2, lq[q], rP[q],
                                                        a[q]);
     } else {
+        float yn[NPIX][3];
+        if (SVBRDF_TARGET_PREFETCH) {
+#pragma unroll
+            for (int q = 0; q < NPIX; ++q)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) yn[q][c] = ld(plane(tgt, c, npix), px[q].off);
+        }
 #pragma unroll UNROLL
         for (int k = 0; k < n_views; ++k) {
             const ViewPos vp = load_view(cam, light, k);
             const float *tk = tgt + (size_t)(3 * k) * npix;
+            const float *tn = tgt + (size_t)(3 * (k + 1 < n_views ? k + 1 : k)) * npix;
 #pragma unroll
             for (int q = 0; q < NPIX; ++q) {
                 float y[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) y[c] = ld(plane(tk, c, npix), px[q].off);
+                for (int c = 0; c < 3; ++c) {
+                    if (SVBRDF_TARGET_PREFETCH) {
+                        y[c] = yn[q][c];
+                        yn[q][c] = ld(plane(tn, c, npix), px[q].off);
+                    } else {
+                        y[c] = ld(plane(tk, c, npix), px[q].off);
+                    }
+                }
                 Geom g;
                 geometry<false>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
                 Shade s;
@@ -533,32 +584,147 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
 #pragma unroll
         for (int c = 0; c < 9; ++c) t[c] = tclamp(traw[q][c], -1.0f, 1.0f);
         decode_backward(t, m[q], a[q], gt);
+        const unsigned o = px[q].off;
+        if (SVBRDF_MOMENT_PREFETCH == 1) {
+            // every load ahead of the first store: one memory latency, not nine
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                mm[q][c] = ld(plane(exp_avg, c, npix), o);
+                vv[q][c] = ld(plane(exp_avg_sq, c, npix), o);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
-            const unsigned o = px[q].off;
-            float pp = traw[q][c], mm = ld(plane(exp_avg, c, npix), o), vv = ld(plane(exp_avg_sq, c, npix), o);
+            float pp = traw[q][c], m1, v1;
+            if (SVBRDF_MOMENT_PREFETCH == 0) {
+                m1 = ld(plane(exp_avg, c, npix), o);
+                v1 = ld(plane(exp_avg_sq, c, npix), o);
+            } else {
+                m1 = mm[q][c];
+                v1 = vv[q][c];
+            }
             // the clamp on read passes the gradient only inside [-1, 1] (inclusive)
             const float g = (traw[q][c] >= -1.0f && traw[q][c] <= 1.0f) ? gt[c] : 0.0f;
-            adam_update(pp, mm, vv, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
+            adam_update(pp, m1, v1, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
             st(plane(tex, c, npix), o, pp);
-            st(plane(exp_avg, c, npix), o, mm);
-            st(plane(exp_avg_sq, c, npix), o, vv);
+            st(plane(exp_avg, c, npix), o, m1);
+            st(plane(exp_avg_sq, c, npix), o, v1);
         }
     }
 }
 
+// Workspace of the fused step: arrival counters, then two buffers of per-block
+// partial sums (float4: loss, dL/dP rgb), used by alternate steps.  Step t
+// writes buffer t & 1 and the reduction of step t's partials happens in a
+// later launch, where they are visible without any cross-block release inside
+// a launch (an agent-scope release per block writes back the whole L2 of the
+// XCD on gfx950).
+//
+// Arrival is counted in two levels so that no address takes more than
+// ceil(blocks / kArrivalGroups) atomics: block b counts itself in group
+// b % kArrivalGroups; the block completing a group counts the group in the top
+// counter.  Each counter has its own 128-byte line.
+constexpr unsigned kArrivalGroups = 64;
+constexpr unsigned kCounterStride = 128 / sizeof(unsigned);
+constexpr size_t kCounterBytes = ((kArrivalGroups + 1) * 128 + 255) / 256 * 256;
+inline size_t partial_buffer_bytes(unsigned nb) { return ((size_t)nb * sizeof(float4) + 255) / 256 * 256; }
+inline size_t fused_workspace_bytes(unsigned nb) { return kCounterBytes + 2 * partial_buffer_bytes(nb); }
+__device__ __forceinline__ float4 *partial_buffer(void *ws, unsigned nb, unsigned parity) {
+    const size_t stride = ((size_t)nb * sizeof(float4) + 255) / 256 * 256;
+    return reinterpret_cast<float4 *>(static_cast<char *>(ws) + kCounterBytes + parity * stride);
+}
+
+// Fixed-order sum of n float4 partials (member .x, and .y .z .w when NQ == 4)
+// by one block of kBlock threads: thread i adds i, i + kBlock, ... in double,
+// then the wave and the block combine in index order.  Every reader of a
+// step's partials (the next step's block 0, the flush, the light finalize)
+// uses this routine, so they agree to the bit.  Result valid in thread 0.
+template <int NQ>
+__device__ void reduce_partials(const float4 *__restrict__ parts, unsigned n, double (&tot)[NQ],
+                                double (*lds)[kBlock / 64]) {
+    double acc[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) acc[c] = 0.0;
+    for (unsigned b0 = threadIdx.x; b0 < n; b0 += 4 * kBlock) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned b = b0 + u * kBlock;
+            q[u] = b < n ? parts[b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc[0] += q[u].x;
+            if constexpr (NQ == 4) { acc[1] += q[u].y; acc[2] += q[u].z; acc[3] += q[u].w; }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        double x = acc[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) lds[c][wave] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        tot[c] = 0.0;
+        if (threadIdx.x == 0)
+            for (int w = 0; w < kBlock / 64; ++w) tot[c] += lds[c][w];
+    }
+    __syncthreads();
+}
+
+// Records the loss of step t (1-based) from its partials: state->loss and
+// loss_history[t - 1].  Thread 0 only.
+__device__ __forceinline__ void record_loss(svbrdf_step_state *st, unsigned t, double total, double count,
+                                            float *loss_history, int history_len) {
+    const float L = (float)(total / count);
+    st->loss = L;
+    if (loss_history && t >= 1 && t <= (unsigned)max(history_len, 0)) loss_history[t - 1] = L;
+}
+
+// Step count, beta powers and bias corrections of the next step.
+__device__ __forceinline__ void advance_step(svbrdf_step_state *st, float step, double beta1, double beta2) {
+    st->step = step;
+    st->beta1_pow *= beta1;
+    st->beta2_pow *= beta2;
+    set_bias_corrections(st);
+}
+
+// One fused iteration over all pixels.
+//  * Every block reads the step state once (Adam scalars, completed steps t),
+//    writes its partial sums to buffer t & 1 and then, from thread 0, counts
+//    itself in the arrival counter.
+//  * Block 0 first reduces the partials of the previous step when they are
+//    pending (state->pending_step == t) and records that step's loss.
+//  * Without light optimisation the block that arrives last (all blocks have
+//    read the state by then) advances the state for the next launch, marks
+//    step t + 1 pending and re-arms the counter: one launch per iteration.
+//  * With light optimisation the light power of the next step depends on the
+//    full dL/dP of this one, so a one-block finalize follows instead.
 template <int LOSS, bool OPT_LIGHT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_FUSED_WAVES))) void fused_step_kernel(
     float *__restrict__ tex, float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
     const float *__restrict__ tgt, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, float *__restrict__ lpow, double lr, double beta1, double beta2,
-    double eps, const svbrdf_step_state *__restrict__ st, float *__restrict__ partials) {
+    double eps, svbrdf_step_state *st, void *ws, float *loss_history, int history_len) {
     constexpr int NPIX = SVBRDF_FUSED_NPIX;
     __shared__ float lds[kBlock / 64];
     const unsigned npix = (unsigned)res * (unsigned)res;
+    const double count = (double)n_views * 3.0 * (double)npix;
     // d mean / d element, times d out^... / d ln img = 1/2.2 (folded once)
-    const float K2 = (float)(2.0 / ((double)n_views * 3.0 * (double)npix) / 2.2);
+    const float K2 = (float)(2.0 / count / 2.2);
     const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+    const float step_f = st->step;
+    const unsigned t = (unsigned)step_f;
+    if (blockIdx.x == 0 && t >= 1 && st->pending_step == t) {
+        __shared__ double ldsd[1][kBlock / 64];
+        double tot[1];
+        reduce_partials<1>(partial_buffer(ws, gridDim.x, (t - 1) & 1u), gridDim.x, tot, ldsd);
+        if (threadIdx.x == 0) record_loss(st, t, tot[0], count, loss_history, history_len);
+    }
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
     Pix px[NPIX];
 #pragma unroll
@@ -572,71 +738,49 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     else
         fused_pixels<LOSS, OPT_LIGHT, false, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, npix,
                                                                          n_views, cam, light, P, K2, sc, loss, gPs);
-    // per-block partials: loss, dL/dP (plain stores; the finalize kernel that
-    // follows on the same stream reads them)
     const float bl = block_sum(loss, lds);
     float bp[3] = {0.f, 0.f, 0.f};
     if (OPT_LIGHT) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) bp[c] = block_sum(gPs[c], lds);
     }
-    if (threadIdx.x == 0) reinterpret_cast<float4 *>(partials)[blockIdx.x] = make_float4(bl, bp[0], bp[1], bp[2]);
+    if (threadIdx.x != 0) return;
+    partial_buffer(ws, gridDim.x, t & 1u)[blockIdx.x] = make_float4(bl, bp[0], bp[1], bp[2]);
+    if constexpr (!OPT_LIGHT) {
+        unsigned *counters = static_cast<unsigned *>(ws);
+        const unsigned nb = gridDim.x, g = blockIdx.x % kArrivalGroups;
+        const unsigned group_size = (nb - g + kArrivalGroups - 1) / kArrivalGroups;
+        const unsigned n_groups = nb < kArrivalGroups ? nb : kArrivalGroups;
+        unsigned *group = counters + (1 + g) * kCounterStride;
+        if (atomicAdd(group, 1u) != group_size - 1) return;
+        __hip_atomic_store(group, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (atomicAdd(counters, 1u) != n_groups - 1) return;
+        // every block of this launch has read the state
+        __hip_atomic_store(counters, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        advance_step(st, step_f + 1.0f, beta1, beta2);
+        st->pending_step = t + 1;
+    }
 }
 
-// One block: sums the per-block partials of a fused step in a fixed order
-// (bit-reproducible), records the loss, takes the light-power Adam step when
-// it is optimised (one param group with the maps, src/svbrdf.py:50) and
-// advances the step state for the next launch.  Running as a separate launch
-// on the same stream is what makes the partials visible: a cross-block
-// hand-off inside the step kernel would need an agent-scope release per
-// block, which writes back the whole L2 of the XCD on gfx950.
-constexpr int kFinalizeBlock = 1024;
-
+// One block, after a light-optimising step: reduces that step's partials in
+// the fixed order of reduce_partials, records the loss, takes the
+// light-power Adam step (one param group with the maps, src/svbrdf.py:50)
+// and advances the state.
 template <bool OPT_LIGHT>
-__global__ __launch_bounds__(kFinalizeBlock) void finalize_step_kernel(
-    const float *__restrict__ partials, unsigned n_partials, double count, float *__restrict__ lpow, double lr,
-    double beta1, double beta2, double eps, svbrdf_step_state *__restrict__ st, float *__restrict__ loss_history,
-    int history_len) {
+__global__ __launch_bounds__(kBlock) void finalize_step_kernel(void *ws, unsigned n_partials, double count,
+                                                              float *__restrict__ lpow, double lr, double beta1,
+                                                              double beta2, double eps, svbrdf_step_state *st,
+                                                              float *loss_history, int history_len) {
     constexpr int NQ = OPT_LIGHT ? 4 : 1;
-    __shared__ double lds[NQ][kFinalizeBlock / 64];
-    const float4 *parts = reinterpret_cast<const float4 *>(partials);
-    double acc[NQ];
-#pragma unroll
-    for (int c = 0; c < NQ; ++c) acc[c] = 0.0;
-    // four independent loads in flight per thread, fixed summation order
-    for (unsigned b0 = threadIdx.x; b0 < n_partials; b0 += 4 * kFinalizeBlock) {
-        float4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const unsigned b = b0 + u * kFinalizeBlock;
-            q[u] = b < n_partials ? parts[b] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            acc[0] += q[u].x;
-            if constexpr (OPT_LIGHT) { acc[1] += q[u].y; acc[2] += q[u].z; acc[3] += q[u].w; }
-        }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c < NQ; ++c) {
-        double x = acc[c];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if (lane == 0) lds[c][wave] = x;
-    }
-    __syncthreads();
+    __shared__ double lds[NQ][kBlock / 64];
+    const float step_f = st->step;
+    const unsigned t = (unsigned)step_f;
+    double tot[NQ];
+    reduce_partials<NQ>(partial_buffer(ws, n_partials, t & 1u), n_partials, tot, lds);
     if (threadIdx.x != 0) return;
-    double tot[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int c = 0; c < NQ; ++c)
-        for (int w = 0; w < kFinalizeBlock / 64; ++w) tot[c] += lds[c][w];
-    const double step = (double)st->step + 1.0;
-    const float L = (float)(tot[0] / count);
-    st->loss = L;
-    if (loss_history && (double)history_len >= step) loss_history[(int)step - 1] = L;
-    if (OPT_LIGHT) {
-        // Adam on light_pow with the same scalars as the maps (one param group)
+    record_loss(st, t + 1, tot[0], count, loss_history, history_len);
+    if constexpr (OPT_LIGHT) {
+        // Adam on light_pow with the same scalars as the maps
         const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
         for (int c = 0; c < 3; ++c) {
             float pp = lpow[c], mm = st->light_exp_avg[c], vv = st->light_exp_avg_sq[c];
@@ -648,10 +792,23 @@ __global__ __launch_bounds__(kFinalizeBlock) void finalize_step_kernel(
             st->light_grad[c] = g;
         }
     }
-    st->step = (float)step;
-    st->beta1_pow *= beta1;
-    st->beta2_pow *= beta2;
-    set_bias_corrections(st);
+    advance_step(st, step_f + 1.0f, beta1, beta2);
+    st->pending_step = 0u;
+}
+
+// One block: records the loss of the last completed step when its partials
+// are still pending, and clears the mark.
+__global__ __launch_bounds__(kBlock) void flush_step_kernel(const void *ws, unsigned n_partials, double count,
+                                                           svbrdf_step_state *st, float *loss_history,
+                                                           int history_len) {
+    __shared__ double lds[1][kBlock / 64];
+    const unsigned t = (unsigned)st->step;
+    if (t == 0 || st->pending_step != t) return;
+    double tot[1];
+    reduce_partials<1>(partial_buffer(const_cast<void *>(ws), n_partials, (t - 1) & 1u), n_partials, tot, lds);
+    if (threadIdx.x != 0) return;
+    record_loss(st, t, tot[0], count, loss_history, history_len);
+    st->pending_step = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -836,7 +993,7 @@ int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream
 
 size_t svbrdf_fused_step_workspace_bytes(int res) {
     if (res < 1 || res > kMaxRes) return 0;
-    return (size_t)n_blocks(res) * 4 * sizeof(float);
+    return fused_workspace_bytes(n_blocks(res, SVBRDF_FUSED_NPIX));
 }
 
 int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const float *targets_prepared, int res,
@@ -855,16 +1012,15 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
     const size_t need = svbrdf_fused_step_workspace_bytes(res);
     if (!workspace || workspace_bytes < need)
         return fail(SVBRDF_ESPACE, "svbrdf_fused_step: workspace of %zu bytes < %zu needed", workspace_bytes, need);
-    if (((uintptr_t)workspace & 15u) || ((uintptr_t)state & 15u))
-        return fail(SVBRDF_EINVAL, "svbrdf_fused_step: workspace and state must be 16-byte aligned");
+    if (((uintptr_t)workspace & 255u) || ((uintptr_t)state & 15u))
+        return fail(SVBRDF_EINVAL, "svbrdf_fused_step: workspace must be 256-byte and state 16-byte aligned");
     const unsigned nb = n_blocks(res, SVBRDF_FUSED_NPIX);
     hipStream_t s = (hipStream_t)stream;
     svbrdf_step_state *st = (svbrdf_step_state *)state;
-    float *ws = (float *)workspace;
 #define SVBRDF_LAUNCH_STEP(L, O)                                                                             \
     hipLaunchKernelGGL((fused_step_kernel<L, O>), dim3(nb), dim3(kBlock), 0, s, textures, exp_avg, exp_avg_sq, \
                        targets_prepared, res, n_views, size, camera_pos, light_pos, light_pow, lr, beta1,      \
-                       beta2, eps, st, ws)
+                       beta2, eps, st, workspace, loss_history, history_len)
     if (loss_mode == SVBRDF_LOSS_LOG_L2) {
         if (optim_light) SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, true);
         else SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, false);
@@ -872,16 +1028,31 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
         if (optim_light) SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_L2, true);
         else SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_L2, false);
     }
-    if ((rc = check_launch("svbrdf_fused_step"))) return rc;
-    const double count = (double)n_views * 3.0 * (double)res * (double)res;
-    if (optim_light)
-        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kFinalizeBlock), 0, s, ws, nb, count, light_pow, lr,
-                           beta1, beta2, eps, st, loss_history, history_len);
-    else
-        hipLaunchKernelGGL(finalize_step_kernel<false>, dim3(1), dim3(kFinalizeBlock), 0, s, ws, nb, count, light_pow, lr,
-                           beta1, beta2, eps, st, loss_history, history_len);
 #undef SVBRDF_LAUNCH_STEP
+    if ((rc = check_launch("svbrdf_fused_step"))) return rc;
+    if (optim_light) {
+        const double count = (double)n_views * 3.0 * (double)res * (double)res;
+        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kBlock), 0, s, workspace, nb, count, light_pow,
+                           lr, beta1, beta2, eps, st, loss_history, history_len);
+    }
     return check_launch("svbrdf_fused_step");
+}
+
+int svbrdf_fused_step_flush(void *state, int res, int n_views, float *loss_history, int history_len,
+                            const void *workspace, size_t workspace_bytes, void *stream) {
+    if (!state || !workspace) return fail(SVBRDF_EINVAL, "svbrdf_fused_step_flush: null pointer argument");
+    if (res < 1 || res > kMaxRes) return fail(SVBRDF_ESHAPE, "svbrdf_fused_step_flush: res=%d", res);
+    if (n_views < 1 || n_views > kMaxViews)
+        return fail(SVBRDF_ESHAPE, "svbrdf_fused_step_flush: n_views=%d", n_views);
+    const size_t need = svbrdf_fused_step_workspace_bytes(res);
+    if (workspace_bytes < need)
+        return fail(SVBRDF_ESPACE, "svbrdf_fused_step_flush: workspace of %zu bytes < %zu needed", workspace_bytes,
+                    need);
+    const double count = (double)n_views * 3.0 * (double)res * (double)res;
+    hipLaunchKernelGGL(flush_step_kernel, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, workspace,
+                       n_blocks(res, SVBRDF_FUSED_NPIX), count, (svbrdf_step_state *)state, loss_history,
+                       history_len);
+    return check_launch("svbrdf_fused_step_flush");
 }
 
 // ---------------------------------------------------------------------------

@@ -8,9 +8,10 @@ launch per iteration (:class:`FusedStep`).
 
 ``FusedStep`` owns the device-resident pieces of one optimisation problem:
 the prepared targets, Adam's moment buffers, the step state and the
-workspace for per-block partial sums.  Its ``step()`` is graph-capturable
-(no host scalars change between iterations); ``capture()`` records one
-iteration in a HIP graph and ``replay()`` runs it.
+workspace (per-block partial sums of the last step and arrival counters).
+Its ``step()`` is graph-capturable (no host scalars change between
+iterations); ``capture(steps)`` records that many iterations in one HIP
+graph and ``replay()`` runs them.
 """
 import ctypes
 import datetime
@@ -26,7 +27,8 @@ LOSSES = {"l2": _lib.LOSS_L2, "log_l2": _lib.LOSS_LOG_L2, "logl2": _lib.LOSS_LOG
 # layout of svbrdf_step_state (include/svbrdf_hip.h)
 _STATE_DTYPE = np.dtype([("beta1_pow", "<f8"), ("beta2_pow", "<f8"),
                          ("inv_bias_correction1", "<f8"), ("inv_bias_correction2_sqrt", "<f4"),
-                         ("step", "<f4"), ("loss", "<f4"), ("reserved", "<u4", 3),
+                         ("step", "<f4"), ("loss", "<f4"), ("pending_step", "<u4"),
+                         ("reserved", "<u4", 2),
                          ("light_exp_avg", "<f4", 4), ("light_exp_avg_sq", "<f4", 4),
                          ("light_grad", "<f4", 4)])
 
@@ -83,11 +85,14 @@ class FusedStep:
         nstate = _lib.query("svbrdf_step_state_bytes")
         self.state = torch.zeros((nstate + 15) // 16 * 4, dtype=torch.float32, device=dev)
         nws = _lib.query("svbrdf_fused_step_workspace_bytes", self.res)
-        self.workspace = torch.empty((nws + 15) // 16 * 4, dtype=torch.float32, device=dev)
+        # zero-filled once: it carries the arrival counter and the last
+        # step's partial sums from one call to the next
+        self.workspace = torch.zeros((nws + 15) // 16 * 4, dtype=torch.float32, device=dev)
         self.history = torch.zeros(max(int(history), 0), dtype=torch.float32, device=dev)
         _lib.call("svbrdf_step_state_init", _lib.ptr(self.state), self.beta1, self.beta2, s)
         self.steps_launched = 0
         self.graph = None
+        self.graph_steps = 0
 
     def step(self):
         """Launch one fused iteration on the current stream (asynchronous)."""
@@ -101,8 +106,11 @@ class FusedStep:
                   _lib.stream_handle(self.textures.device))
         self.steps_launched += 1
 
-    def capture(self):
-        """Record one iteration into a HIP graph (torch.cuda.CUDAGraph)."""
+    def capture(self, steps=1):
+        """Record `steps` consecutive iterations into one HIP graph
+        (torch.cuda.CUDAGraph); every replay() then runs that many."""
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(device=self.textures.device)
         s.wait_stream(torch.cuda.current_stream())
@@ -110,23 +118,36 @@ class FusedStep:
             g.capture_begin()
             r = self.renderer
             hist = self.history if self.history.numel() else None
-            _lib.call("svbrdf_fused_step", _lib.ptr(self.textures), _lib.ptr(self.exp_avg),
-                      _lib.ptr(self.exp_avg_sq), _lib.ptr(self.targets), self.res, self.n_views, r.size,
-                      _lib.ptr(r.camera_pos), _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode,
-                      int(self.optim_light), self.lr, self.beta1, self.beta2, self.eps, _lib.ptr(self.state),
-                      _lib.ptr(hist), self.history.numel(), _lib.ptr(self.workspace),
-                      self.workspace.numel() * 4, ctypes.c_void_p(s.cuda_stream))
+            for _ in range(steps):
+                _lib.call("svbrdf_fused_step", _lib.ptr(self.textures), _lib.ptr(self.exp_avg),
+                          _lib.ptr(self.exp_avg_sq), _lib.ptr(self.targets), self.res, self.n_views, r.size,
+                          _lib.ptr(r.camera_pos), _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode,
+                          int(self.optim_light), self.lr, self.beta1, self.beta2, self.eps,
+                          _lib.ptr(self.state), _lib.ptr(hist), self.history.numel(),
+                          _lib.ptr(self.workspace), self.workspace.numel() * 4, ctypes.c_void_p(s.cuda_stream))
             g.capture_end()
         torch.cuda.current_stream().wait_stream(s)
         self.graph = g
+        self.graph_steps = steps
         return g
 
     def replay(self):
         self.graph.replay()
-        self.steps_launched += 1
+        self.steps_launched += self.graph_steps
+
+    def flush(self):
+        """Records the last step's loss if it is still pending (asynchronous).
+
+        A step launch reduces the previous step's partial sums, so the loss of
+        the most recent step needs this one-block launch before it is read."""
+        hist = self.history if self.history.numel() else None
+        _lib.call("svbrdf_fused_step_flush", _lib.ptr(self.state), self.res, self.n_views, _lib.ptr(hist),
+                  self.history.numel(), _lib.ptr(self.workspace), self.workspace.numel() * 4,
+                  _lib.stream_handle(self.textures.device))
 
     def read_state(self):
-        """Host copy of the device step state (synchronises)."""
+        """Host copy of the device step state, losses flushed (synchronises)."""
+        self.flush()
         raw = self.state.detach().cpu().numpy().view(np.uint8)[:_STATE_DTYPE.itemsize]
         return raw.view(_STATE_DTYPE)[0]
 
@@ -136,11 +157,14 @@ class FusedStep:
 
     def losses(self):
         """Per-step losses recorded so far (synchronises)."""
+        self.flush()
         n = min(self.steps_launched, self.history.numel())
         return self.history[:n].cpu().numpy()
 
     def state_dict(self):
-        """Exact resume point: maps, Adam moments and the step state."""
+        """Exact resume point: maps, Adam moments and the step state (flushed,
+        so it does not refer to partial sums held in this workspace)."""
+        self.flush()
         return {"textures": self.textures.detach().clone(), "exp_avg": self.exp_avg.clone(),
                 "exp_avg_sq": self.exp_avg_sq.clone(), "state": self.state.clone(),
                 "light_pow": self.light_pow.detach().clone()}
@@ -151,8 +175,12 @@ class FusedStep:
             self.exp_avg.copy_(sd["exp_avg"])
             self.exp_avg_sq.copy_(sd["exp_avg_sq"])
             self.state.copy_(sd["state"])
+            # partial sums do not travel with a state: nothing is pending here
+            self.state.view(torch.int32)[_STATE_DTYPE.fields["pending_step"][1] // 4] = 0
             if self.optim_light:
                 self.light_pow.copy_(sd["light_pow"])
+        raw = sd["state"].detach().cpu().numpy().view(np.uint8)[:_STATE_DTYPE.itemsize]
+        self.steps_launched = int(raw.view(_STATE_DTYPE)[0]["step"])
 
 
 class SvbrdfOptim:

@@ -24,7 +24,8 @@ def lib(pkg):
 def test_header_declares_expected_entry_points():
     names = declared_functions()
     for n in ("svbrdf_render", "svbrdf_render_backward", "svbrdf_fused_step", "svbrdf_prepare_targets",
-              "svbrdf_step_state_init", "svbrdf_abi_version", "svbrdf_last_error"):
+              "svbrdf_step_state_init", "svbrdf_abi_version", "svbrdf_last_error",
+              "svbrdf_fused_step_flush"):
         assert n in names
 
 
@@ -35,13 +36,15 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 1
+    assert lib.svbrdf_abi_version() == 2
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
     assert lib.svbrdf_max_views() >= 16
-    # one float4 partial per 256-pixel block
-    assert lib.svbrdf_fused_step_workspace_bytes(1024) == 1024 * 1024 // 256 * 16
+    # 65 arrival counters on 128-byte lines (8448 bytes) + two buffers of one
+    # float4 partial per 256-pixel block, each rounded up to 256 bytes
+    assert lib.svbrdf_fused_step_workspace_bytes(1024) == 8448 + 2 * (1024 * 1024 // 256 * 16)
+    assert lib.svbrdf_fused_step_workspace_bytes(1) == 8448 + 2 * 256
     assert lib.svbrdf_fused_step_workspace_bytes(0) == 0
     assert lib.svbrdf_render_backward_workspace_bytes(13) == 1 * 16
 
@@ -73,7 +76,7 @@ def test_backward_requires_workspace_for_light_gradient(lib):
 def _step(lib, **kw):
     a = dict(tex=_fake(1), m=_fake(2), v=_fake(3), tgt=_fake(4), res=64, views=9, cam=_fake(5), light=_fake(6),
              pow_=_fake(7), mode=1, light_opt=0, lr=0.01, b1=0.9, b2=0.999, eps=1e-8, state=_fake(8),
-             hist=None, hlen=0, ws=_fake(9), wsb=64 * 64 // 256 * 16)
+             hist=None, hlen=0, ws=_fake(9), wsb=lib.svbrdf_fused_step_workspace_bytes(64))
     a.update(kw)
     return lib.svbrdf_fused_step(a["tex"], a["m"], a["v"], a["tgt"], a["res"], a["views"], ctypes.c_float(17.0),
                                  a["cam"], a["light"], a["pow_"], a["mode"], a["light_opt"],
@@ -88,7 +91,18 @@ def test_fused_step_validation(lib):
     assert _step(lib, state=None) == -1
     assert _step(lib, wsb=16) == -3
     assert _step(lib, ws=ctypes.c_void_p(0x1004)) == -1      # misaligned workspace
+    assert _step(lib, ws=ctypes.c_void_p(0x1080)) == -1      # only 128-byte aligned
     assert _step(lib, res=0) == -2
+
+
+def test_flush_validation(lib):
+    need = lib.svbrdf_fused_step_workspace_bytes(64)
+    assert lib.svbrdf_fused_step_flush(None, 64, 9, None, 0, _fake(9), need, None) == -1
+    assert lib.svbrdf_fused_step_flush(_fake(8), 64, 9, None, 0, None, need, None) == -1
+    assert lib.svbrdf_fused_step_flush(_fake(8), 0, 9, None, 0, _fake(9), need, None) == -2
+    assert lib.svbrdf_fused_step_flush(_fake(8), 64, 0, None, 0, _fake(9), need, None) == -2
+    assert lib.svbrdf_fused_step_flush(_fake(8), 64, 9, None, 0, _fake(9), need - 1, None) == -3
+    assert b"svbrdf_fused_step_flush" in lib.svbrdf_last_error()
 
 
 def test_prepare_and_state_validation(lib):

@@ -139,6 +139,73 @@ def test_deterministic(pkg):
         np.testing.assert_array_equal(a, b)
 
 
+def raw_state(pkg, fs):
+    """Device step state without the flush that read_state() performs."""
+    dt = pkg.optim._STATE_DTYPE
+    return fs.state.detach().cpu().numpy().view(np.uint8)[:dt.itemsize].view(dt)[0]
+
+
+@pytest.mark.parametrize("res", [16, 200, 1024])
+def test_single_launch_records_losses_one_step_late_and_flush_completes(pkg, res):
+    """Without light optimisation a step is one launch: step t's partial sums
+    are reduced by the launch of step t + 1 (block 0) or by the flush.  Both
+    use one fixed-order reduction, so flushing after every step gives the
+    same losses bit for bit, and the arrival counters are back at 0 after
+    every launch."""
+    gt, _, size, cam, light, P = scene(pkg, res, 9, seed=8)
+    init = pkg.synthetic.init_textures(gt, seed=2)
+    target = M.render(gt, res, size, cam, light, P, np.float32)
+    fa, ta = make_problem(pkg, init, target, res, size, cam, light, P, "log_l2", history=8)
+    fb, tb = make_problem(pkg, init, target, res, size, cam, light, P, "log_l2", history=8)
+    for _ in range(3):
+        fa.step()
+        fb.step()
+        fb.flush()
+    torch.cuda.synchronize()
+    hist = fa.history.cpu().numpy()
+    assert np.all(hist[:2] > 0) and hist[2] == 0.0
+    st = raw_state(pkg, fa)
+    assert st["step"] == 3.0 and st["pending_step"] == 3 and st["loss"] == hist[1]
+    # all 65 arrival counters (one per 128-byte line) are back at zero
+    assert int(fa.workspace.view(torch.int32)[:65 * 32].abs().sum().item()) == 0
+    assert raw_state(pkg, fb)["pending_step"] == 0
+    la = fa.losses()
+    np.testing.assert_array_equal(la, fb.losses())
+    assert la[2] > 0 and fa.loss() == la[2]
+    assert raw_state(pkg, fa)["pending_step"] == 0
+    # a second flush changes nothing
+    fa.flush()
+    np.testing.assert_array_equal(fa.losses(), la)
+    np.testing.assert_array_equal(ta.cpu().numpy(), tb.cpu().numpy())
+
+
+def test_state_dict_resume_in_a_fresh_workspace(pkg):
+    res = 96
+    gt, _, size, cam, light, P = scene(pkg, res, 9, seed=9)
+    init = pkg.synthetic.init_textures(gt, seed=3)
+    target = M.render(gt, res, size, cam, light, P, np.float32)
+    fa, ta = make_problem(pkg, init, target, res, size, cam, light, P, "l2", history=8)
+    for _ in range(5):
+        fa.step()
+    fb, _ = make_problem(pkg, init, target, res, size, cam, light, P, "l2", history=8)
+    for _ in range(2):
+        fb.step()
+    sd = fb.state_dict()
+    fc, tc = make_problem(pkg, init, target, res, size, cam, light, P, "l2", history=8)
+    fc.load_state_dict(sd)
+    for _ in range(3):
+        fc.step()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ta.cpu().numpy(), tc.cpu().numpy())
+    la = fa.losses()
+    hc = fc.losses()
+    # the resumed run records steps 3..5 at their own indices
+    assert len(hc) == 5
+    np.testing.assert_array_equal(hc[2:5], la[2:5])
+    assert np.all(hc[:2] == 0.0)
+    assert fc.read_state()["step"] == 5.0
+
+
 def test_gradient_against_fp64_finite_differences(pkg):
     """Config 2's check: 64 random pixels x 9 parameters, FP64 central
     differences (eps 1e-4) of the oracle loss; kinks are skipped."""
