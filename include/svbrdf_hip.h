@@ -69,7 +69,8 @@ int svbrdf_abi_version(void);
 const char *svbrdf_last_error(void);
 size_t svbrdf_step_state_bytes(void);
 
-/* Largest resolution and view count accepted. */
+/* Largest resolution and view count accepted (8192: the 9 planes of a map
+ * must fit one 32-bit buffer range). */
 int svbrdf_max_res(void);
 int svbrdf_max_views(void);
 

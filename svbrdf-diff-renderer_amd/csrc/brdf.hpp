@@ -303,12 +303,12 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
 // scaled by dF/dspe = 1 - 2^(A+B) once per pixel (coloc_finish).
 __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, const float gf[3], float gnl,
                                                Accum &a) {
-    float W = 0.f;
+    float W;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         a.S[c] += gf[c];
         a.T[c] += gf[c] * s.DGq;
-        W += gf[c] * m.Fc[c];
+        W = c == 0 ? gf[c] * m.Fc[c] : W + gf[c] * m.Fc[c];
     }
     // f2 = D F G / q, q = 4 nv nl + eps
     const float Wi = W * s.iq;

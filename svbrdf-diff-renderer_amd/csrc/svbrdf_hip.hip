@@ -30,7 +30,7 @@ using namespace svbrdf;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxRes = 16384;
+constexpr int kMaxRes = 8192;   // 9 planes of res^2 floats within a 32-bit buffer range
 constexpr int kMaxViews = 1024;
 
 thread_local std::string g_last_error;
@@ -99,25 +99,30 @@ struct Pix {
     float px, py;    // position on the plane
 };
 
-// Plane access through buffer instructions: a resource descriptor for plane
-// c of a tensor (base, byte size; four SGPRs, wave-uniform) plus the lane's
-// 32-bit byte offset, which every plane shares.  Offsets past the plane read
-// 0 and drop stores in hardware.  Word 3 = 0x00020000 selects the 32-bit data
-// format that untyped gfx9 buffer accesses need.
-struct Plane {
+// Plane access through buffer instructions: ONE resource descriptor per
+// planar tensor (base, byte size of all its planes; four SGPRs, wave-uniform),
+// the plane chosen by the instruction's scalar offset (c * plane bytes) and the
+// pixel by the lane's 32-bit byte offset, which every plane shares.  One
+// descriptor per tensor instead of one per plane keeps the step kernel's
+// scalar registers within budget (27 per-plane descriptors spilled to vector
+// lanes).  Word 3 = 0x00020000 selects the 32-bit data format that untyped
+// gfx9 buffer accesses need.  All planes of a tensor must fit the 32-bit
+// range: 9 * res^2 * 4 bytes, hence kMaxRes = 8192.
+struct Tensor {
     __amdgpu_buffer_rsrc_t r;
+    unsigned stride;   // bytes per plane
 };
-__device__ __forceinline__ Plane plane(const float *base, unsigned c, unsigned npix) {
-    Plane p;
-    p.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base) + (size_t)c * npix, 0, (int)(npix * 4u),
-                                            0x00020000);
-    return p;
+__device__ __forceinline__ Tensor tensor(const float *base, unsigned n_planes, unsigned npix) {
+    Tensor t;
+    t.stride = npix * 4u;
+    t.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), 0, (int)(n_planes * t.stride), 0x00020000);
+    return t;
 }
-__device__ __forceinline__ float ld(Plane p, unsigned byte_off) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p.r, byte_off, 0, 0));
+__device__ __forceinline__ float ld(const Tensor &t, unsigned c, unsigned byte_off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(t.r, byte_off, (int)(c * t.stride), 0));
 }
-__device__ __forceinline__ void st(Plane p, unsigned byte_off, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p.r, byte_off, 0, 0);
+__device__ __forceinline__ void st(const Tensor &t, unsigned c, unsigned byte_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), t.r, byte_off, (int)(c * t.stride), 0);
 }
 
 __device__ __forceinline__ Pix pixel_at(unsigned p, unsigned npix, int res, float size) {
@@ -171,16 +176,17 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
                                               const float *__restrict__ light, const float P[3],
                                               float *__restrict__ out) {
     Maps m[NPIX];
+    const Tensor T = tensor(tex, 9, npix);
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
         float t[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) t[c] = ld(plane(tex, c, npix), px[q].off);
+        for (int c = 0; c < 9; ++c) t[c] = ld(T, c, px[q].off);
         decode(t, m[q]);
     }
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
-        float *ok = out + (size_t)(3 * k) * npix;
+        const Tensor O = tensor(out + (size_t)(3 * k) * npix, 3, npix);
 #pragma unroll
         for (int q = 0; q < NPIX; ++q) {
             float img[3];
@@ -201,7 +207,7 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const float ic = tclamp(img[c], kEps, 1.0f);
-                    st(plane(ok, c, npix), px[q].off, fexp2(flog2(ic) * kInvGamma));
+                    st(O, c, px[q].off, fexp2(flog2(ic) * kInvGamma));
                 }
             }
         }
@@ -234,15 +240,15 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
                                                Accum &a) {
     float t[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) t[c] = ld(plane(tex, c, npix), px.off);
+    for (int c = 0; c < 9; ++c) t[c] = ld(tensor(tex, 9, npix), c, px.off);
     Maps m;
     decode(t, m);
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
-        const float *gk = gout + (size_t)(3 * k) * npix;
+        const Tensor G = tensor(gout + (size_t)(3 * k) * npix, 3, npix);
         float go[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) go[c] = px.valid ? ld(plane(gk, c, npix), px.off) : 0.0f;
+        for (int c = 0; c < 3; ++c) go[c] = px.valid ? ld(G, c, px.off) : 0.0f;
         // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
         auto dimg = [&](float img, float g) {
             const float ic = tclamp(img, kEps, 1.0f);
@@ -278,7 +284,7 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
     decode_backward(t, m, a, gt);
     if (px.valid) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) st(plane(gtex, c, npix), px.off, gt[c]);
+        for (int c = 0; c < 9; ++c) st(tensor(gtex, 9, npix), c, px.off, gt[c]);
     }
 }
 
@@ -394,7 +400,7 @@ __device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st,
 #define SVBRDF_FUSED_UNROLL 1     // views per loop trip
 #endif
 #ifndef SVBRDF_FUSED_WAVES
-#define SVBRDF_FUSED_WAVES 1      // minimum waves per SIMD the register allocator must allow
+#define SVBRDF_FUSED_WAVES 5      // minimum waves per SIMD the register allocator must allow
 #endif
 #ifndef SVBRDF_MOMENT_PREFETCH
 // When the Adam moments are loaded: 0 = per channel next to its stores (the
@@ -443,7 +449,7 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
     float yn[3];
     if (SVBRDF_TARGET_PREFETCH) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) yn[c] = ld(plane(tgt, c, npix), px.off);
+        for (int c = 0; c < 3; ++c) yn[c] = ld(tensor(tgt, 3, npix), c, px.off);
     }
 #pragma unroll UNROLL
     for (int k = 0; k < n_views; ++k) {
@@ -452,20 +458,20 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
         if (SVBRDF_TARGET_PREFETCH) {
             // issue view k+1's loads now (the last view re-reads its own, an L1/L2 hit)
             const int kn = k + 1 < n_views ? k + 1 : k;
-            const float *tn = tgt + (size_t)(3 * kn) * npix;
+            const Tensor TN = tensor(tgt + (size_t)(3 * kn) * npix, 3, npix);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 y[c] = yn[c];
-                yn[c] = ld(plane(tn, c, npix), px.off);
+                yn[c] = ld(TN, c, px.off);
             }
         } else {
-            const float *tk = tgt + (size_t)(3 * k) * npix;
+            const Tensor TK = tensor(tgt + (size_t)(3 * k) * npix, 3, npix);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) y[c] = ld(plane(tk, c, npix), px.off);
+            for (int c = 0; c < 3; ++c) y[c] = ld(TK, c, px.off);
         }
         ShadeC s;
         shade_coloc(m, px.px, px.py, cx, cy, cz, s);
-        float gf[3], rsum = 0.f;
+        float gf[3], rsum;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float f = m.f1[c] + s.DGq * m.Fc[c];
@@ -484,7 +490,7 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
             }
             loss += r * r;
             gf[c] = K2 * rm * frcp(fmaxf(f, 1e-30f));
-            rsum += rm;
+            rsum = c == 0 ? rm : rsum + rm;   // no "0 + x": it is not x for x = -0
             if (OPT_LIGHT) rP[c] += rm;
         }
         backward_coloc(m, s, gf, K2 * rsum * frcp(fmaxf(s.nv, 1e-30f)), a);
@@ -499,6 +505,7 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
                                              const float *__restrict__ cam, const float *__restrict__ light,
                                              const float P[3], float K2, const StepScalars &sc, float &loss,
                                              float (&gP)[3]) {
+    const Tensor TX = tensor(tex, 9, npix), TM = tensor(exp_avg, 9, npix), TV = tensor(exp_avg_sq, 9, npix);
     float traw[NPIX][9];
     float mm[NPIX][9], vv[NPIX][9];
     Maps m[NPIX];
@@ -508,12 +515,12 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) traw[q][c] = ld(plane(tex, c, npix), px[q].off);
+        for (int c = 0; c < 9; ++c) traw[q][c] = ld(TX, c, px[q].off);
         if (SVBRDF_MOMENT_PREFETCH == 2) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
-                mm[q][c] = ld(plane(exp_avg, c, npix), px[q].off);
-                vv[q][c] = ld(plane(exp_avg_sq, c, npix), px[q].off);
+                mm[q][c] = ld(TM, c, px[q].off);
+                vv[q][c] = ld(TV, c, px[q].off);
             }
         }
         float t[9];
@@ -535,13 +542,13 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
 #pragma unroll
             for (int q = 0; q < NPIX; ++q)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) yn[q][c] = ld(plane(tgt, c, npix), px[q].off);
+                for (int c = 0; c < 3; ++c) yn[q][c] = ld(tensor(tgt, 3, npix), c, px[q].off);
         }
 #pragma unroll UNROLL
         for (int k = 0; k < n_views; ++k) {
             const ViewPos vp = load_view(cam, light, k);
-            const float *tk = tgt + (size_t)(3 * k) * npix;
-            const float *tn = tgt + (size_t)(3 * (k + 1 < n_views ? k + 1 : k)) * npix;
+            const Tensor TK = tensor(tgt + (size_t)(3 * k) * npix, 3, npix);
+            const Tensor TN = tensor(tgt + (size_t)(3 * (k + 1 < n_views ? k + 1 : k)) * npix, 3, npix);
 #pragma unroll
             for (int q = 0; q < NPIX; ++q) {
                 float y[3];
@@ -549,9 +556,9 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
                 for (int c = 0; c < 3; ++c) {
                     if (SVBRDF_TARGET_PREFETCH) {
                         y[c] = yn[q][c];
-                        yn[q][c] = ld(plane(tn, c, npix), px[q].off);
+                        yn[q][c] = ld(TN, c, px[q].off);
                     } else {
-                        y[c] = ld(plane(tk, c, npix), px[q].off);
+                        y[c] = ld(TK, c, px[q].off);
                     }
                 }
                 Geom g;
@@ -589,16 +596,16 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
             // every load ahead of the first store: one memory latency, not nine
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
-                mm[q][c] = ld(plane(exp_avg, c, npix), o);
-                vv[q][c] = ld(plane(exp_avg_sq, c, npix), o);
+                mm[q][c] = ld(TM, c, o);
+                vv[q][c] = ld(TV, c, o);
             }
         }
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
             float pp = traw[q][c], m1, v1;
             if (SVBRDF_MOMENT_PREFETCH == 0) {
-                m1 = ld(plane(exp_avg, c, npix), o);
-                v1 = ld(plane(exp_avg_sq, c, npix), o);
+                m1 = ld(TM, c, o);
+                v1 = ld(TV, c, o);
             } else {
                 m1 = mm[q][c];
                 v1 = vv[q][c];
@@ -606,9 +613,9 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
             // the clamp on read passes the gradient only inside [-1, 1] (inclusive)
             const float g = (traw[q][c] >= -1.0f && traw[q][c] <= 1.0f) ? gt[c] : 0.0f;
             adam_update(pp, m1, v1, g, sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps, sc.neg_step);
-            st(plane(tex, c, npix), o, pp);
-            st(plane(exp_avg, c, npix), o, m1);
-            st(plane(exp_avg_sq, c, npix), o, v1);
+            st(TX, c, o, pp);
+            st(TM, c, o, m1);
+            st(TV, c, o, v1);
         }
     }
 }
@@ -815,8 +822,11 @@ __global__ __launch_bounds__(kBlock) void flush_step_kernel(const void *ws, unsi
 // Batched persistent step.  A grid sized to the device pulls tiles from an
 // atomic ticket counter; the counter is the only state blocks share, so no
 // fence is needed, and every block leaves once its ticket passes n_tiles.
+#ifndef SVBRDF_BATCH_WAVES
+#define SVBRDF_BATCH_WAVES 4      // minimum waves per SIMD for the batched kernel's register allocation (128 VGPRs, no spills at one pixel per thread)
+#endif
 template <int LOSS, int NPIX, int UNROLL>
-__global__ __launch_bounds__(kBlock) void batch_step_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 1 ? SVBRDF_BATCH_WAVES : 1))) void batch_step_kernel(
     const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
     const unsigned *__restrict__ tile_mat, unsigned n_tiles, int tile_pixels, unsigned *__restrict__ queue,
     float4 *__restrict__ partials, double lr, double beta1, double beta2, double eps,
