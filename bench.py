@@ -24,9 +24,11 @@ Printed JSON line (one):
   batched           config 3: 64 materials of uneven size in one persistent
                     launch (autotuned), against a per-material loop
   composed          config 4: render adjoint under a VGG-shaped feature loss
+  codec             the map / target codec kernels (8-bit maps and images <->
+                    tensors, Lanczos-4 resize, stage transition) at the same shape
 
     python bench.py [--steps K] [--warmup W] [--res 1024] [--views 9]
-                    [--no-cpu] [--no-torch] [--no-batch] [--no-compose]
+                    [--no-cpu] [--no-torch] [--no-batch] [--no-compose] [--no-codec]
                     [--cpu-seconds 20] [--graph-steps 10 | --eager]
 """
 import argparse
@@ -250,6 +252,40 @@ def batch_leg(pkg, dev, steps=20, warmup=3):
             "speedup_vs_loop": round(ms_loop / ms, 3)}
 
 
+def codec_leg(pkg, dev, res=1024, views=9, iters=20):
+    """The map / target codec around the path (src/imageio.py, src/svbrdf.py:
+    168-237) at the bench shape: bytes moved / time, against 8 TB/s."""
+    from svbrdf_amd import imageio as io
+    from svbrdf_amd import svbrdf_io as sio
+    rng = np.random.default_rng(9)
+    u8 = torch.from_numpy(rng.integers(0, 256, (views, res, res, 3), dtype=np.uint8)).to(dev)
+    tex = (torch.rand((1, 9, res, res), generator=torch.Generator().manual_seed(9)) * 2 - 1).to(dev)
+    half = tex[:, :, ::2, ::2].contiguous()
+    maps = sio.maps_u8_from_textures(tex)
+    small = sio.maps_u8_from_textures(half)
+    npx = res * res
+    cases = {
+        "load_images (svbrdf_images_from_u8, srgb)": (lambda: io.decode(u8, "srgb"), views * npx * (3 + 12)),
+        "load_textures (svbrdf_textures_from_u8)": (lambda: sio.textures_from_maps_u8(*maps, res), npx * (10 + 36)),
+        "save_textures (svbrdf_textures_to_u8)": (lambda: sio.maps_u8_from_textures(tex), npx * (36 + 10)),
+        f"resize {res // 2}->{res} x3 (svbrdf_resize_lanczos4_u8)": (
+            lambda: io.imresize(small[0], (res, res)), (npx // 4) * 3 + npx * 3),
+        f"stage transition {res // 2}->{res} (save + 4 resizes + load)": (
+            lambda: sio.reload_textures(half, res), None),
+    }
+    out = {}
+    for name, (fn, nbytes) in cases.items():
+        for _ in range(3):
+            fn()
+        ms = time_events(fn, iters)
+        rec = {"ms": round(ms, 5)}
+        if nbytes:
+            rec.update(bytes=nbytes, achieved_GBs=round(nbytes / (ms * 1e-3) / 1e9, 1),
+                       frac=round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        out[name] = rec
+    return out
+
+
 def compose_leg(pkg, dev, res=256, views=9, iters=100):
     """Config 4: render -> VGG-19-shaped random-init conv stack -> Gram L2 on
     5 layers + pixel L2 -> backward (HIP adjoint) -> torch Adam, 100 steps."""
@@ -347,6 +383,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-material batched leg (config 3)")
     ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (config 4)")
+    ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="iterations captured per HIP graph (graph launches carry a fixed cost each)")
@@ -428,6 +465,8 @@ def main():
         out["batched"] = batch_leg(pkg, dev)
     if not args.no_compose:
         out["composed"] = compose_leg(pkg, dev)
+    if not args.no_codec:
+        out["codec"] = codec_leg(pkg, dev, res, views)
     if not args.no_torch:
         try:
             tb = torch_gpu_baseline(prob, res, views, dev)

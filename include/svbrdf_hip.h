@@ -151,6 +151,63 @@ int svbrdf_fused_step_flush(void *state, int res, int n_views, float *loss_histo
                             size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Map / target codec: the per-pixel work between PNG files and the tensors
+ * above (src/imageio.py:11-76, src/svbrdf.py:126-132,168-237).  The PNG
+ * container (inflate / deflate, scanline filters) stays on the host; these
+ * take and produce 8-bit images in device memory, RGB-ordered and
+ * channel-interleaved ([H, W, C] bytes), as the files store them.
+ * ------------------------------------------------------------------------ */
+
+/* imresize (src/imageio.py:75-76): cv2.resize(..., INTER_LANCZOS4) of an
+ * 8-bit [src_h, src_w, channels] image (channels 1..4) to [dst_h, dst_w,
+ * channels]: 8x8 taps, replicated border, 11-bit fixed-point weights,
+ * rounded and saturated; a same-size resize is a copy.  Sides up to 16384. */
+int svbrdf_resize_lanczos4_u8(const unsigned char *src, int src_h, int src_w, int channels,
+                              unsigned char *dst, int dst_h, int dst_w, void *stream);
+
+/* imread flags (src/imageio.py:26-47). */
+enum svbrdf_imread_flag {
+    SVBRDF_IMREAD_NONE = 0,    /* u / 255 per channel                               */
+    SVBRDF_IMREAD_SRGB = 1,    /* 3 channels, u / 255                               */
+    SVBRDF_IMREAD_ROUGH = 2,   /* 1 channel, or the float mean of 3, of u / 255      */
+    SVBRDF_IMREAD_NORMAL = 3,  /* 3 channels, u / 255 * 2 - 1 divided by its 2-norm  */
+};
+
+/* imread after the resize (src/imageio.py:11-49; SvbrdfIO.load_images_th,
+ * src/svbrdf.py:209-221, is n "srgb" images): n images [n, height, width,
+ * channels] bytes -> [n, C', height, width] float32, C' = channels, 1 for
+ * SVBRDF_IMREAD_ROUGH, 3 for SVBRDF_IMREAD_NORMAL. */
+int svbrdf_images_from_u8(const unsigned char *images, int n, int height, int width, int channels,
+                          int flag, float *out, void *stream);
+
+/* SvbrdfIO.load_textures_th (src/svbrdf.py:168-183) after the resizes: the
+ * diffuse, normal and specular maps [res, res, 3] and the roughness map
+ * [res, res, rgh_channels] (1, or 3 averaged) -> textures [9, res, res]:
+ * u / 255 * 2 - 1 for diffuse / specular / roughness, the normal decoded as
+ * u / 255 * 2 - 1 and divided by its 2-norm, channels 0:3 diffuse, 3:5
+ * normal xy, 5 roughness, 6:9 specular. */
+int svbrdf_textures_from_u8(const unsigned char *dif, const unsigned char *nom,
+                            const unsigned char *spe, const unsigned char *rgh,
+                            int rgh_channels, int res, float *textures, void *stream);
+
+/* SvbrdfIO.save_textures_th (src/svbrdf.py:185-207) before the PNG encode:
+ * textures [9, res, res] -> dif / nom / spe [res, res, 3] and rgh
+ * [res, res] bytes: (t + 1) / 2 clipped to [0, 1] (normal: reconstructed
+ * as SvbrdfIO.reconstruct_normal, clipped to [-1, 1], then (n + 1) / 2),
+ * times 255, truncated (src/imageio.py:52-72). */
+int svbrdf_textures_to_u8(const float *textures, int res, unsigned char *dif,
+                          unsigned char *nom, unsigned char *spe, unsigned char *rgh,
+                          void *stream);
+
+/* imwrite before the PNG encode (src/imageio.py:52-72; SvbrdfIO.save_images_th,
+ * src/svbrdf.py:223-237, is n "srgb" images): [n, channels, height, width]
+ * float32 -> [n, height, width, channels] bytes = (v * 255) truncated, with v
+ * clipped to [0, 1] (SRGB, ROUGH with 1 channel, and NONE, whose unclipped
+ * reference cast is only defined there) or (clip(v, -1, 1) + 1) / 2 (NORMAL). */
+int svbrdf_images_to_u8(const float *images, int n, int height, int width, int channels,
+                        int flag, unsigned char *out, void *stream);
+
+/* ------------------------------------------------------------------------
  * Batched step: many independent materials (uneven resolution and view count)
  * in one persistent launch.  The reference runs one material per process
  * (src/scripts.py:67-97); this is the same per-pixel iteration applied to

@@ -20,6 +20,7 @@ EHIP = -4
 
 LOSS_L2 = 0
 LOSS_LOG_L2 = 1
+IMREAD_NONE, IMREAD_SRGB, IMREAD_ROUGH, IMREAD_NORMAL = 0, 1, 2, 3
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
@@ -45,6 +46,14 @@ SIGNATURES = {
     # batched step (svbrdf_material tables are passed as pointers)
     "svbrdf_fused_step_flush": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
                                                ctypes.c_size_t, _vp]),
+    "svbrdf_resize_lanczos4_u8": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                                 ctypes.c_int, _vp]),
+    "svbrdf_images_from_u8": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _vp, _vp]),
+    "svbrdf_textures_from_u8": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "svbrdf_textures_to_u8": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "svbrdf_images_to_u8": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, _vp, _vp]),
     "svbrdf_batch_tile_count": (ctypes.c_size_t, [_vp, ctypes.c_int, ctypes.c_int]),
     "svbrdf_batch_plan": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_size_t]),
     "svbrdf_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),

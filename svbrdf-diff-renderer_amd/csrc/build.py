@@ -15,8 +15,8 @@ CSRC = pathlib.Path(__file__).resolve().parent
 PKG = CSRC.parent
 ROOT = PKG.parent
 LIB = PKG / "libsvbrdf_hip.so"
-SOURCES = [CSRC / "svbrdf_hip.hip"]
-HEADERS = [CSRC / "brdf.hpp", ROOT / "include" / "svbrdf_hip.h"]
+SOURCES = [CSRC / "svbrdf_hip.hip", CSRC / "codec.hip", CSRC / "common.hip"]
+HEADERS = [CSRC / "brdf.hpp", CSRC / "common.hpp", ROOT / "include" / "svbrdf_hip.h"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -46,6 +46,10 @@ def build(asm=False, verbose=False):
         subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-S",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                         f"-I{ROOT / 'include'}", str(SOURCES[0]), "-o", str(out / "svbrdf_hip.s")],
+                       check=True)
+        subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-S",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        f"-I{ROOT / 'include'}", str(SOURCES[1]), "-o", str(out / "codec.s")],
                        check=True)
     return LIB
 

@@ -1,0 +1,358 @@
+// codec.hip -- gfx950 kernels for the map / target codec on either side of the
+// optimisation path, and their extern "C" entry points (include/svbrdf_hip.h).
+//
+// Reference behaviour (src/imageio.py:11-76, src/svbrdf.py:126-132,168-237):
+// PNG files hold 8-bit maps and images; loading resizes the 8-bit image with
+// cv2.INTER_LANCZOS4, divides by 255 and applies the per-map flag; saving
+// clips, scales by 255 and truncates to 8 bits.  The PNG container itself
+// (inflate / deflate, scanline filters) is sequential byte work and stays on
+// the host; every per-pixel step is here.
+//
+// Kernels (byte and float elementwise work, HBM-bound):
+//   resize_lanczos4_kernel  8-bit Lanczos-4 resample, 8x8 taps, integer-exact
+//   images_from_u8_kernel   [n, h, w, C] u8 -> [n, C', h, w] f32   C B read, 4 C' B written / px
+//   textures_from_u8_kernel 4 maps u8 -> [9, R, R] f32             10-12 B read, 36 B written / px
+//   textures_to_u8_kernel   [9, R, R] f32 -> 4 maps u8             36 B read, 10 B written / px
+//   images_to_u8_kernel     [n, C, h, w] f32 -> [n, h, w, C] u8    4 C B read, C B written / px
+//
+// Float arithmetic follows the reference's float32 numpy / torch op order one
+// rounding per op: contraction into fused multiply-adds is off in this file,
+// and `/` and sqrtf are correctly rounded (hipcc's default fp32 divide / sqrt
+// lowering; the __fsqrt_rn intrinsic is the native approximation instead).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "common.hpp"
+#include "../../include/svbrdf_hip.h"
+
+#pragma clang fp contract(off)
+
+using svbrdf_internal::check_launch;
+using svbrdf_internal::fail;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 16;                 // resize: 16 x 16 output pixels per block
+constexpr int kMaxSide = 16384;           // largest 8-bit image side accepted
+constexpr int kMaxRes = 8192;             // as the renderer's maximum resolution
+constexpr int kCoefBits = 11;             // fixed-point bits of the resize weights
+
+// ---------------------------------------------------------------------------
+// Lanczos-4 resize of 8-bit images, as OpenCV's INTER_LANCZOS4 for 8U data
+// (modules/imgproc/src/resize.cpp).  Destination index d samples source
+// coordinate f = (d + 0.5) * scale - 0.5, scale = 1 / (dst / src) in double,
+// rounded to float; taps floor(f) - 3 .. floor(f) + 4, border replicated.
+// Weights: the window sinc(t) sinc(t/4) at t_i = frac + 3 - i, which is
+// (-1)^i sin(a - i pi/4) / t_i^2 up to a factor common to all taps
+// (a = (frac + 3) pi / 4); normalised in float, scaled by 2^11, rounded
+// half-to-even.  Sums are exact integers; the result is (sum + 2^21) >> 22
+// saturated to [0, 255].
+struct AxisPlan {
+    int first;   // source index of tap 0 (before clamping)
+    int w[8];
+};
+
+__device__ void lanczos4_plan(int d, int n_src, int n_dst, AxisPlan &p) {
+    const double scale = 1.0 / ((double)n_dst / (double)n_src);
+    const float f = (float)(((double)d + 0.5) * scale - 0.5);
+    const float s = floorf(f);
+    const float x = f - s;
+    p.first = (int)s - 3;
+    const float xp3 = x + 3.0f;
+    const double a0 = -(double)xp3 * M_PI * 0.25;
+    const double s0 = sin(a0), c0 = cos(a0);
+    constexpr double kS45 = 0.70710678118654752440;
+    // (-1)^(i+1) (cos(i pi/4), sin(i pi/4))
+    constexpr double kA[8] = {-1.0, kS45, 0.0, -kS45, 1.0, -kS45, 0.0, kS45};
+    constexpr double kB[8] = {0.0, kS45, -1.0, kS45, 0.0, -kS45, 1.0, -kS45};
+    float wf[8];
+    float total = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float t = xp3 - (float)i;
+        if (fabsf(t) >= 1e-6f) {
+            const double y = -(double)t * M_PI * 0.25;
+            wf[i] = (float)((kA[i] * s0 + kB[i] * c0) / (y * y));
+        } else {
+            wf[i] = 1e30f;   // the unit impulse after normalisation
+        }
+        total = total + wf[i];
+    }
+    const float inv = 1.0f / total;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float v = rintf((wf[i] * inv) * (float)(1 << kCoefBits));
+        p.w[i] = (int)fminf(fmaxf(v, -32768.0f), 32767.0f);
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(kBlock) void resize_lanczos4_kernel(const uint8_t *__restrict__ src, int sh, int sw,
+                                                                 uint8_t *__restrict__ dst, int dh, int dw) {
+    __shared__ AxisPlan plan_x[kTile], plan_y[kTile];
+    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+    const int t = threadIdx.x;
+    if (t < kTile) {
+        if (x0 + t < dw) lanczos4_plan(x0 + t, sw, dw, plan_x[t]);
+    } else if (t < 2 * kTile) {
+        if (y0 + t - kTile < dh) lanczos4_plan(y0 + t - kTile, sh, dh, plan_y[t - kTile]);
+    }
+    __syncthreads();
+    const int tx = t % kTile, ty = t / kTile;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= dw || y >= dh) return;
+    const AxisPlan &ax = plan_x[tx], &ay = plan_y[ty];
+    int cols[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) cols[j] = min(max(ax.first + j, 0), sw - 1);
+    int acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int row = min(max(ay.first + k, 0), sh - 1);
+        const uint8_t *r = src + (size_t)row * sw * C;
+        int h[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) h[c] = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int c = 0; c < C; ++c) h[c] += (int)r[(size_t)cols[j] * C + c] * ax.w[j];
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] += h[c] * ay.w[k];
+    }
+    uint8_t *o = dst + ((size_t)y * dw + x) * C;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int v = (acc[c] + (1 << (2 * kCoefBits - 1))) >> (2 * kCoefBits);
+        o[c] = (uint8_t)min(max(v, 0), 255);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel conversions.  u / 255 is the correctly rounded float quotient, as
+// numpy's float32 division (src/imageio.py:17-18).
+__device__ __forceinline__ float unit(uint8_t u) { return (float)u / 255.0f; }
+
+// (v * 255).astype("uint8") of a value already clipped to [0, 1]: truncation.
+__device__ __forceinline__ uint8_t to_u8(float v) { return (uint8_t)(int)(v * 255.0f); }
+
+// NaN-preserving clip, as numpy's clip.
+__device__ __forceinline__ float clip(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// imread's per-flag decode (src/imageio.py:27-47) of one pixel of an 8-bit
+// image already at its final size.
+__device__ __forceinline__ float decode_rough(const uint8_t *px, int channels) {
+    if (channels == 3) {
+        // np.mean over the channel axis: (a + b) + c, then / 3
+        const float s = (unit(px[0]) + unit(px[1])) + unit(px[2]);
+        return s / 3.0f;
+    }
+    return unit(px[0]);
+}
+
+// "normal": im * 2 - 1 divided by np.linalg.norm (sqrt of the sequential sum
+// of squares)
+__device__ __forceinline__ void decode_normal(const uint8_t *px, float n[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n[c] = unit(px[c]) * 2.0f - 1.0f;
+    const float norm = sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
+}
+
+// imread after the resize, for n images of h x w pixels: [n, h, w, C] bytes ->
+// [n, C', h, w] float32 with C' = C (no flag, "srgb"), 1 ("rough") or 3
+// ("normal").
+__global__ __launch_bounds__(kBlock) void images_from_u8_kernel(const uint8_t *__restrict__ in, size_t total,
+                                                                unsigned npix, int channels, int flag,
+                                                                float *__restrict__ out) {
+    const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= total) return;
+    const size_t img = p / npix, q = p - img * npix;
+    const uint8_t *src = in + p * (size_t)channels;
+    if (flag == SVBRDF_IMREAD_ROUGH) {
+        out[img * npix + q] = decode_rough(src, channels);
+    } else if (flag == SVBRDF_IMREAD_NORMAL) {
+        float n[3];
+        decode_normal(src, n);
+        float *dst = out + img * 3 * (size_t)npix + q;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dst[(size_t)c * npix] = n[c];
+    } else {
+        float *dst = out + img * (size_t)channels * npix + q;
+        for (int c = 0; c < channels; ++c) dst[(size_t)c * npix] = unit(src[c]);
+    }
+}
+
+// load_textures_th (src/svbrdf.py:168-183) with imread flags (src/imageio.py:
+// 27-47): maps already at res x res, RGB bytes (roughness 1 or 3 channels).
+__global__ __launch_bounds__(kBlock) void textures_from_u8_kernel(const uint8_t *__restrict__ dif,
+                                                                  const uint8_t *__restrict__ nom,
+                                                                  const uint8_t *__restrict__ spe,
+                                                                  const uint8_t *__restrict__ rgh, int rgh_channels,
+                                                                  unsigned npix, float *__restrict__ tex) {
+    const unsigned p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    float out[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        out[c] = unit(dif[3 * (size_t)p + c]) * 2.0f - 1.0f;
+        out[6 + c] = unit(spe[3 * (size_t)p + c]) * 2.0f - 1.0f;
+    }
+    out[5] = decode_rough(rgh + (size_t)rgh_channels * p, rgh_channels) * 2.0f - 1.0f;
+    float n[3];
+    decode_normal(nom + 3 * (size_t)p, n);
+    out[3] = n[0];
+    out[4] = n[1];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) tex[(size_t)c * npix + p] = out[c];
+}
+
+// save_textures_th (src/svbrdf.py:185-207) with imwrite flags (src/imageio.py:
+// 52-72): [9, R, R] float32 -> dif / nom / spe [R, R, 3] and rgh [R, R] bytes.
+// The normal is SvbrdfIO.reconstruct_normal (src/svbrdf.py:126-132): z from
+// clamp(x^2 + y^2, 0, 1), divided by the 2-norm summed as (x^2 + y^2) + z^2.
+__global__ __launch_bounds__(kBlock) void textures_to_u8_kernel(const float *__restrict__ tex, unsigned npix,
+                                                                uint8_t *__restrict__ dif, uint8_t *__restrict__ nom,
+                                                                uint8_t *__restrict__ spe, uint8_t *__restrict__ rgh) {
+    const unsigned p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    auto half = [](float t) { return (t + 1.0f) / 2.0f; };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dif[3 * (size_t)p + c] = to_u8(clip(half(tex[(size_t)c * npix + p]), 0.0f, 1.0f));
+        spe[3 * (size_t)p + c] = to_u8(clip(half(tex[(size_t)(6 + c) * npix + p]), 0.0f, 1.0f));
+    }
+    rgh[p] = to_u8(clip(half(tex[(size_t)5 * npix + p]), 0.0f, 1.0f));
+    const float x = clip(tex[(size_t)3 * npix + p], -1.0f, 1.0f);
+    const float y = clip(tex[(size_t)4 * npix + p], -1.0f, 1.0f);
+    const float xy = x * x + y * y;
+    const float z = sqrtf(1.0f - clip(xy, 0.0f, 1.0f));
+    const float norm = sqrtf(xy + z * z);
+    const float n[3] = {x / norm, y / norm, z / norm};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nom[3 * (size_t)p + c] = to_u8(half(clip(n[c], -1.0f, 1.0f)));
+}
+
+// imwrite before the PNG encode (src/imageio.py:52-72), for n images of
+// h x w pixels: [n, C, h, w] float32 -> [n, h, w, C] bytes.  "srgb" and
+// "rough" clip to [0, 1]; "normal" clips to [-1, 1] and maps (n + 1) / 2;
+// without a flag the reference casts unclipped values, which is only defined
+// inside [0, 1], so values are clipped there too.
+__global__ __launch_bounds__(kBlock) void images_to_u8_kernel(const float *__restrict__ in, size_t total,
+                                                              unsigned npix, int channels, int flag,
+                                                              uint8_t *__restrict__ out) {
+    const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= total) return;
+    const size_t img = p / npix, q = p - img * npix;
+    const float *src = in + img * (size_t)channels * npix + q;
+    for (int c = 0; c < channels; ++c) {
+        const float v = src[(size_t)c * npix];
+        out[p * channels + c] = flag == SVBRDF_IMREAD_NORMAL ? to_u8((clip(v, -1.0f, 1.0f) + 1.0f) / 2.0f)
+                                                             : to_u8(clip(v, 0.0f, 1.0f));
+    }
+}
+
+unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int check_image_args(const char *what, const void *a, const void *b, int n, int height, int width, int channels,
+                     int flag) {
+    if (!a || !b) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (n < 1 || height < 1 || width < 1 || height > kMaxSide || width > kMaxSide)
+        return fail(SVBRDF_ESHAPE, "%s: n=%d height=%d width=%d", what, n, height, width);
+    if (flag < SVBRDF_IMREAD_NONE || flag > SVBRDF_IMREAD_NORMAL)
+        return fail(SVBRDF_EINVAL, "%s: unknown flag %d", what, flag);
+    if (channels < 1 || channels > 4) return fail(SVBRDF_EINVAL, "%s: channels=%d outside [1, 4]", what, channels);
+    if ((flag == SVBRDF_IMREAD_SRGB || flag == SVBRDF_IMREAD_NORMAL) && channels != 3)
+        return fail(SVBRDF_EINVAL, "%s: this flag needs a 3 channel image, got %d", what, channels);
+    if (flag == SVBRDF_IMREAD_ROUGH && channels != 1 && channels != 3)
+        return fail(SVBRDF_EINVAL, "%s: a roughness image has 1 or 3 channels, got %d", what, channels);
+    return SVBRDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int svbrdf_resize_lanczos4_u8(const unsigned char *src, int src_h, int src_w, int channels, unsigned char *dst,
+                              int dst_h, int dst_w, void *stream) {
+    const char *what = "svbrdf_resize_lanczos4_u8";
+    if (!src || !dst) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (channels < 1 || channels > 4) return fail(SVBRDF_EINVAL, "%s: channels=%d outside [1, 4]", what, channels);
+    if (src_h < 1 || src_w < 1 || dst_h < 1 || dst_w < 1 || src_h > kMaxSide || src_w > kMaxSide ||
+        dst_h > kMaxSide || dst_w > kMaxSide)
+        return fail(SVBRDF_ESHAPE, "%s: sizes %dx%d -> %dx%d outside [1, %d]", what, src_h, src_w, dst_h, dst_w,
+                    kMaxSide);
+    hipStream_t s = (hipStream_t)stream;
+    if (src_h == dst_h && src_w == dst_w) {
+        // cv2.resize returns the source unchanged at equal size
+        if (hipMemcpyAsync(dst, src, (size_t)src_h * src_w * channels, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return fail(SVBRDF_EHIP, "%s: copy failed", what);
+        return SVBRDF_OK;
+    }
+    const dim3 grid((dst_w + kTile - 1) / kTile, (dst_h + kTile - 1) / kTile);
+    switch (channels) {
+        case 1: hipLaunchKernelGGL(resize_lanczos4_kernel<1>, grid, dim3(kBlock), 0, s, src, src_h, src_w, dst, dst_h, dst_w); break;
+        case 2: hipLaunchKernelGGL(resize_lanczos4_kernel<2>, grid, dim3(kBlock), 0, s, src, src_h, src_w, dst, dst_h, dst_w); break;
+        case 3: hipLaunchKernelGGL(resize_lanczos4_kernel<3>, grid, dim3(kBlock), 0, s, src, src_h, src_w, dst, dst_h, dst_w); break;
+        default: hipLaunchKernelGGL(resize_lanczos4_kernel<4>, grid, dim3(kBlock), 0, s, src, src_h, src_w, dst, dst_h, dst_w); break;
+    }
+    return check_launch(what);
+}
+
+int svbrdf_images_from_u8(const unsigned char *images, int n, int height, int width, int channels, int flag,
+                          float *out, void *stream) {
+    const char *what = "svbrdf_images_from_u8";
+    int rc = check_image_args(what, images, out, n, height, width, channels, flag);
+    if (rc) return rc;
+    const unsigned npix = (unsigned)height * (unsigned)width;
+    const size_t total = (size_t)n * npix;
+    hipLaunchKernelGGL(images_from_u8_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, (hipStream_t)stream, images,
+                       total, npix, channels, flag, out);
+    return check_launch(what);
+}
+
+int svbrdf_textures_from_u8(const unsigned char *dif, const unsigned char *nom, const unsigned char *spe,
+                            const unsigned char *rgh, int rgh_channels, int res, float *textures, void *stream) {
+    const char *what = "svbrdf_textures_from_u8";
+    if (!dif || !nom || !spe || !rgh || !textures) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (rgh_channels != 1 && rgh_channels != 3)
+        return fail(SVBRDF_EINVAL, "%s: roughness map must have 1 or 3 channels, got %d", what, rgh_channels);
+    if (res < 1 || res > kMaxRes) return fail(SVBRDF_ESHAPE, "%s: res=%d outside [1, %d]", what, res, kMaxRes);
+    const unsigned npix = (unsigned)res * (unsigned)res;
+    hipLaunchKernelGGL(textures_from_u8_kernel, dim3(blocks_for(npix)), dim3(kBlock), 0, (hipStream_t)stream, dif,
+                       nom, spe, rgh, rgh_channels, npix, textures);
+    return check_launch(what);
+}
+
+int svbrdf_textures_to_u8(const float *textures, int res, unsigned char *dif, unsigned char *nom,
+                          unsigned char *spe, unsigned char *rgh, void *stream) {
+    const char *what = "svbrdf_textures_to_u8";
+    if (!textures || !dif || !nom || !spe || !rgh) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (res < 1 || res > kMaxRes) return fail(SVBRDF_ESHAPE, "%s: res=%d outside [1, %d]", what, res, kMaxRes);
+    const unsigned npix = (unsigned)res * (unsigned)res;
+    hipLaunchKernelGGL(textures_to_u8_kernel, dim3(blocks_for(npix)), dim3(kBlock), 0, (hipStream_t)stream, textures,
+                       npix, dif, nom, spe, rgh);
+    return check_launch(what);
+}
+
+int svbrdf_images_to_u8(const float *images, int n, int height, int width, int channels, int flag,
+                        unsigned char *out, void *stream) {
+    const char *what = "svbrdf_images_to_u8";
+    int rc = check_image_args(what, images, out, n, height, width, channels, flag);
+    if (rc) return rc;
+    if (flag == SVBRDF_IMREAD_ROUGH && channels != 1)
+        return fail(SVBRDF_EINVAL, "%s: a roughness image is written with 1 channel", what);
+    const unsigned npix = (unsigned)height * (unsigned)width;
+    const size_t total = (size_t)n * npix;
+    hipLaunchKernelGGL(images_to_u8_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, (hipStream_t)stream, images,
+                       total, npix, channels, flag, out);
+    return check_launch(what);
+}
+
+}  // extern "C"

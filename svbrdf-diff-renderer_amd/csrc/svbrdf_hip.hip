@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "brdf.hpp"
+#include "common.hpp"
 #include "../../include/svbrdf_hip.h"
 
 using namespace svbrdf;
@@ -33,24 +34,8 @@ constexpr int kBlock = 256;
 constexpr int kMaxRes = 8192;   // 9 planes of res^2 floats within a 32-bit buffer range
 constexpr int kMaxViews = 1024;
 
-thread_local std::string g_last_error;
-
-int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
-int fail(int code, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
-
-int check_launch(const char *what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(SVBRDF_EHIP, "%s: launch failed: %s", what, hipGetErrorString(e));
-    return SVBRDF_OK;
-}
+using svbrdf_internal::check_launch;
+using svbrdf_internal::fail;
 
 int check_common(const char *what, const void *tex, int res, int n_views, const void *cam,
                  const void *light, const void *pow_) {
@@ -932,7 +917,7 @@ int check_variant(const char *what, int loss_mode, int npix, int unroll) {
 extern "C" {
 
 int svbrdf_abi_version(void) { return SVBRDF_ABI_VERSION; }
-const char *svbrdf_last_error(void) { return g_last_error.c_str(); }
+const char *svbrdf_last_error(void) { return svbrdf_internal::last_error(); }
 size_t svbrdf_step_state_bytes(void) { return sizeof(svbrdf_step_state); }
 int svbrdf_max_res(void) { return kMaxRes; }
 int svbrdf_max_views(void) { return kMaxViews; }

@@ -1,0 +1,180 @@
+"""8-bit image I/O for maps and targets: src/imageio.py with the per-pixel
+work on the GPU.
+
+The reference reads and writes PNG files with cv2 (BGR order in memory) and
+does the per-pixel work in numpy on the host.  Here the PNG container --
+inflate / deflate and the scanline filters, sequential byte work -- is
+Pillow's, on the host; the arrays keep the file's RGB order, so the
+reference's BGR<->RGB swaps cancel.  Everything per pixel runs in
+libsvbrdf_hip.so:
+
+* :func:`imresize` -- ``cv2.resize(..., INTER_LANCZOS4)`` of an 8-bit image
+  (src/imageio.py:75-76), ``svbrdf_resize_lanczos4_u8``;
+* :func:`imread` -- src/imageio.py:11-49: resize, ``u / 255`` and the
+  "srgb" / "rough" / "normal" flags, ``svbrdf_images_from_u8``; returns a
+  float32 CUDA tensor [H, W, C] (or [H, W] for "rough"), as the reference
+  returns a numpy array of that shape;
+* :func:`imwrite` -- src/imageio.py:52-72: clip per flag, ``* 255``,
+  truncate, ``svbrdf_images_to_u8``;
+* :func:`imconcat`, :func:`img9to1`, :func:`tex4to1` -- the montages of
+  src/imageio.py:79-113.
+
+Only 8-bit PNGs are handled (the reference's data are 8-bit; its uint16
+branch, src/imageio.py:19-20, has no 16-bit RGB counterpart in Pillow).
+"""
+import pathlib
+
+import numpy as np
+import torch
+from PIL import Image
+
+from . import _lib
+
+FLAGS = {None: _lib.IMREAD_NONE, "srgb": _lib.IMREAD_SRGB, "rough": _lib.IMREAD_ROUGH,
+         "normal": _lib.IMREAD_NORMAL}
+
+
+def _flag(flag):
+    try:
+        return FLAGS[flag]
+    except KeyError:
+        raise ValueError(f"unknown flag {flag!r}; expected one of {list(FLAGS)}") from None
+
+
+def read_png(filename):
+    """The stored bytes of an 8-bit PNG: [H, W] (grey) or [H, W, C], RGB order."""
+    with Image.open(str(filename)) as im:
+        if im.mode in ("I", "I;16", "I;16B", "F"):
+            raise ValueError(f"{filename}: only 8-bit images are supported, got mode {im.mode}")
+        if im.mode == "P":
+            im = im.convert("RGBA" if "transparency" in im.info else "RGB")
+        if im.mode not in ("L", "LA", "RGB", "RGBA"):
+            im = im.convert("RGB")
+        return np.asarray(im, dtype=np.uint8).copy()
+
+
+def write_png(filename, arr):
+    """Writes [H, W] or [H, W, 3 | 4] uint8 (RGB order) as an 8-bit PNG."""
+    arr = np.ascontiguousarray(arr, dtype=np.uint8)
+    Image.fromarray(arr).save(str(filename), format="PNG")
+
+
+def _device(device):
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError("svbrdf_amd image I/O runs on the GPU only; got device %r" % (str(dev),))
+    return dev
+
+
+def to_device_u8(arr, device="cuda"):
+    """uint8 array or tensor -> contiguous uint8 CUDA tensor."""
+    if isinstance(arr, torch.Tensor):
+        t = arr
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.uint8))
+    return t.to(device=_device(device), dtype=torch.uint8).contiguous()
+
+
+def _hwc(t):
+    return t if t.dim() == 3 else t.unsqueeze(-1)
+
+
+def imresize(im, dim):
+    """cv2.resize(im, dim, interpolation=cv2.INTER_LANCZOS4) for an 8-bit
+    [H, W] or [H, W, C] CUDA tensor; dim = (width, height) as cv2 takes it."""
+    if im.dtype != torch.uint8 or im.device.type != "cuda":
+        raise ValueError("imresize takes a uint8 CUDA tensor (8-bit images)")
+    src = _hwc(im.contiguous())
+    h, w, c = src.shape
+    dw, dh = int(dim[0]), int(dim[1])
+    out = torch.empty((dh, dw, c), dtype=torch.uint8, device=im.device)
+    _lib.call("svbrdf_resize_lanczos4_u8", _lib.ptr(src), h, w, c, _lib.ptr(out), dh, dw,
+              _lib.stream_handle(im.device))
+    return out if im.dim() == 3 else out[:, :, 0]
+
+
+def decode(u8, flag=None):
+    """imread's arithmetic after the resize on [n, H, W, C] (or [H, W, C])
+    uint8 CUDA images -> float32 [n, C', H, W] planar (C' = 1 for "rough")."""
+    single = u8.dim() == 3
+    x = u8.unsqueeze(0) if single else u8
+    if x.dim() != 4:
+        raise ValueError("decode takes [n, H, W, C] or [H, W, C] images")
+    x = x.contiguous()
+    n, h, w, c = x.shape
+    f = _flag(flag)
+    cout = 1 if f == _lib.IMREAD_ROUGH else (3 if f == _lib.IMREAD_NORMAL else c)
+    out = torch.empty((n, cout, h, w), dtype=torch.float32, device=x.device)
+    _lib.call("svbrdf_images_from_u8", _lib.ptr(x), n, h, w, c, f, _lib.ptr(out), _lib.stream_handle(x.device))
+    return out[0] if single else out
+
+
+def encode(images, flag=None):
+    """imwrite's arithmetic before the PNG encode on [n, C, H, W] (or
+    [C, H, W]) float32 CUDA images -> uint8 [n, H, W, C]."""
+    single = images.dim() == 3
+    x = images.unsqueeze(0) if single else images
+    x = x.detach().to(torch.float32).contiguous()
+    n, c, h, w = x.shape
+    out = torch.empty((n, h, w, c), dtype=torch.uint8, device=x.device)
+    _lib.call("svbrdf_images_to_u8", _lib.ptr(x), n, h, w, c, _flag(flag), _lib.ptr(out),
+              _lib.stream_handle(x.device))
+    return out[0] if single else out
+
+
+def imread(filename, flag=None, dim=None, device="cuda"):
+    """src/imageio.py:11-49.  Returns a float32 CUDA tensor [H, W, C] ([H, W]
+    for "rough" and for grey images without a flag)."""
+    u = read_png(filename)
+    f = _flag(flag)
+    if f in (_lib.IMREAD_SRGB, _lib.IMREAD_NORMAL) and (u.ndim != 3 or u.shape[2] != 3):
+        raise ValueError(f"{filename} should be a 3 channel image")
+    if f == _lib.IMREAD_ROUGH and not (u.ndim == 2 or (u.ndim == 3 and u.shape[2] == 3)):
+        raise ValueError(f"{filename} should be a 3 or 1 channel image")
+    t = to_device_u8(u, device)
+    if dim is not None:
+        t = imresize(t, dim)
+    out = decode(_hwc(t), flag)           # [C', H, W]
+    if f == _lib.IMREAD_ROUGH or (u.ndim == 2 and out.shape[0] == 1):
+        return out[0]
+    return out.permute(1, 2, 0)
+
+
+def imwrite(im, filename, flag=None, dim=None):
+    """src/imageio.py:52-72 for a float [H, W, C] or [H, W] image (CUDA tensor
+    or array).  The reference's optional float resize before writing is not
+    used on its hot path and is not provided."""
+    if dim is not None:
+        raise NotImplementedError("imwrite(dim=...) (a float-image resize) is not provided")
+    t = torch.as_tensor(im)
+    if t.device.type != "cuda":
+        t = t.to(_device("cuda"))
+    chw = t.unsqueeze(0) if t.dim() == 2 else t.permute(2, 0, 1)
+    u = encode(chw, flag)
+    write_png(filename, u[:, :, 0].cpu().numpy() if t.dim() == 2 else u.cpu().numpy())
+
+
+def imconcat(im_list, size=(2, 2)):
+    """src/imageio.py:79-90: rows of `size[0]` images, `size[1]` rows."""
+    w, h = size
+    rows = [np.concatenate(im_list[i * w:(i + 1) * w], axis=1) for i in range(h)]
+    return np.concatenate(rows, axis=0)
+
+
+def img9to1(folder):
+    """src/imageio.py:93-99: the 9 views 00..08.png as a 3 x 3 montage all.png.
+    The reference passes every pixel through imread's u / 255 and imwrite's
+    (v * 255).astype(uint8); in float32 that returns each of the 256 levels
+    unchanged, so the montage holds the stored bytes."""
+    folder = pathlib.Path(folder)
+    write_png(folder / "all.png", imconcat([read_png(folder / f"{i:02d}.png") for i in range(9)], (3, 3)))
+
+
+def tex4to1(folder):
+    """src/imageio.py:102-113: normal, diffuse, specular, roughness (grey made
+    3-channel) as a 2 x 2 montage tex.png (stored bytes, as in img9to1)."""
+    folder = pathlib.Path(folder)
+    ims = [read_png(folder / f"{n}.png") for n in ("nom", "dif", "spe", "rgh")]
+    if ims[3].ndim == 2:
+        ims[3] = np.repeat(ims[3][:, :, None], 3, axis=2)
+    write_png(folder / "tex.png", imconcat(ims))

@@ -1,0 +1,154 @@
+"""Map / target codec on the GPU against the oracle (oracle/codec_np.py):
+byte outputs bit for bit, float outputs bit for bit (the kernels follow the
+oracle's float32 op order with no fused multiply-adds and correctly rounded
+division / square root), through the C ABI and through SvbrdfIO with real
+PNG files."""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import codec_np as C
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def smooth_u8(rng, shape):
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    base = 127 + 100 * np.sin(xx / 6.0 + rng.uniform(0, 3)) * np.cos(yy / 4.0)
+    if len(shape) == 3:
+        base = np.repeat(base[:, :, None], shape[2], 2) + rng.uniform(-20, 20, (1, 1, shape[2]))
+    return np.clip(base + rng.normal(0, 25, base.shape), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("shape,dst", [((64, 64, 3), (256, 256)), ((256, 256, 3), (64, 64)), ((50, 31, 1), (77, 64)),
+                                       ((33, 47, 3), (20, 90)), ((128, 96, 4), (200, 150)), ((37, 53, 3), (37, 53)),
+                                       ((1, 1, 3), (5, 7)), ((300, 200, 2), (150, 401)), ((512, 512, 3), (1024, 1024))])
+def test_resize_matches_oracle_bit_for_bit(pkg, shape, dst):
+    from svbrdf_amd import imageio as io
+    print(f"resize {shape} -> {dst}", flush=True)
+    rng = np.random.default_rng(sum(shape) + sum(dst))
+    im = smooth_u8(rng, shape)
+    out = io.imresize(torch.from_numpy(im).to(DEV), (dst[1], dst[0])).cpu().numpy()
+    ref = C.resize_lanczos4_u8(im, *dst)
+    assert out.shape == ref.shape
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("flag,channels", [(None, 3), (None, 1), ("srgb", 3), ("rough", 3), ("rough", 1),
+                                           ("normal", 3), (None, 4)])
+def test_decode_flags_match_oracle(pkg, flag, channels):
+    from svbrdf_amd import imageio as io
+    rng = np.random.default_rng(channels)
+    ims = rng.integers(0, 256, (3, 17, 29, channels), dtype=np.uint8)
+    out = io.decode(torch.from_numpy(ims).to(DEV), flag).cpu().numpy()
+    for k in range(3):
+        src = ims[k] if channels > 1 else ims[k, :, :, 0]
+        ref = C.imread_u8(src, flag)
+        ref = ref[None] if ref.ndim == 2 else ref.transpose(2, 0, 1)
+        np.testing.assert_array_equal(out[k], ref)
+
+
+@pytest.mark.parametrize("flag,channels", [("srgb", 3), ("rough", 1), ("normal", 3), (None, 3)])
+def test_encode_flags_match_oracle(pkg, flag, channels):
+    from svbrdf_amd import imageio as io
+    rng = np.random.default_rng(10 + channels)
+    x = rng.uniform(-1.3, 1.3, (2, channels, 19, 23)).astype(np.float32)
+    x[0, 0, 0, :4] = [0.0, 1.0, 0.5, -0.0]
+    out = io.encode(torch.from_numpy(x).to(DEV), flag).cpu().numpy()
+    for k in range(2):
+        v = x[k].transpose(1, 2, 0)
+        if flag == "normal":
+            ref = C.to_u8(((np.clip(v, -1, 1) + np.float32(1)).astype(np.float32) / np.float32(2)).astype(np.float32))
+        else:
+            ref = C.to_u8(np.clip(v, 0, 1))
+        np.testing.assert_array_equal(out[k], ref)
+
+
+@pytest.mark.parametrize("res,rgh_channels", [(64, 1), (64, 3), (33, 3), (1, 1)])
+def test_textures_from_u8_matches_oracle(pkg, res, rgh_channels):
+    from svbrdf_amd.svbrdf_io import textures_from_maps_u8
+    rng = np.random.default_rng(res + rgh_channels)
+    dif, nom, spe = (rng.integers(0, 256, (res, res, 3), dtype=np.uint8) for _ in range(3))
+    rgh = rng.integers(0, 256, (res, res, 3) if rgh_channels == 3 else (res, res), dtype=np.uint8)
+    t = textures_from_maps_u8(*(torch.from_numpy(m).to(DEV) for m in (dif, nom, spe, rgh)), res)
+    np.testing.assert_array_equal(t[0].cpu().numpy(), C.load_textures(dif, nom, spe, rgh, res))
+
+
+@pytest.mark.parametrize("res", [64, 33, 1])
+def test_textures_to_u8_matches_oracle(pkg, res):
+    from svbrdf_amd.svbrdf_io import maps_u8_from_textures
+    rng = np.random.default_rng(100 + res)
+    tex = rng.uniform(-1.2, 1.2, (9, res, res)).astype(np.float32)
+    tex[3:5, 0, 0] = [0.0, 0.0]              # flat normal
+    out = [m.cpu().numpy() for m in maps_u8_from_textures(torch.from_numpy(tex).to(DEV)[None])]
+    ref = C.save_textures(tex)
+    for name, a, b in zip(("dif", "nom", "spe", "rgh"), out, ref):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+def test_reload_textures_is_save_then_load(pkg):
+    rng = np.random.default_rng(7)
+    tex = np.clip(rng.normal(0, 0.5, (9, 64, 64)), -1, 1).astype(np.float32)
+    got = pkg.reload_textures(torch.from_numpy(tex).to(DEV), 128)[0].cpu().numpy()
+    np.testing.assert_array_equal(got, C.load_textures(*C.save_textures(tex), 128))
+
+
+def test_svbrdf_io_files_end_to_end(pkg, tmp_path):
+    from svbrdf_amd import imageio as io
+    rng = np.random.default_rng(11)
+    ref_dir, tgt_dir = tmp_path / "ref", tmp_path / "target"
+    ref_dir.mkdir()
+    tgt_dir.mkdir()
+    maps = {"dif": smooth_u8(rng, (300, 300, 3)), "nom": smooth_u8(rng, (300, 300, 3)),
+            "spe": smooth_u8(rng, (300, 300, 3)), "rgh": smooth_u8(rng, (300, 300))}
+    for k, m in maps.items():
+        io.write_png(ref_dir / f"{k}.png", m)
+    idx = [0, 2, 5]
+    views = {i: smooth_u8(rng, (200, 200, 3)) for i in idx}
+    for i, v in views.items():
+        io.write_png(tgt_dir / f"{i:02d}.png", v)
+    cam = np.round(rng.uniform(-10, 10, (6, 3)), 3).tolist()
+    (tmp_path / "cfg.json").write_text(json.dumps({
+        "reference_dir": "ref", "target_dir": "target", "optimize_dir": "opt", "rerender_dir": "rr",
+        "im_size": 17.0, "idx": idx, "camera_pos": cam, "light_pos": cam, "light_pow": [1500, 1500, 1500]}))
+    sio = pkg.SvbrdfIO(tmp_path / "cfg.json", DEV)
+    assert sio.n_of_imgs == 3 and sio.im_size == 17.0 and sio.target_dir == tgt_dir
+    np.testing.assert_array_equal(sio.cl[0].cpu().numpy(), np.float32(cam)[idx])
+    tex = sio.load_textures_th(sio.reference_dir, 256)
+    assert tex.shape == (1, 9, 256, 256)
+    np.testing.assert_array_equal(tex[0].cpu().numpy(),
+                                  C.load_textures(maps["dif"], maps["nom"], maps["spe"], maps["rgh"], 256))
+    imgs = sio.load_images_th(sio.target_dir, 128)
+    np.testing.assert_array_equal(imgs.cpu().numpy(), C.load_images([views[i] for i in idx], 128))
+    sio.save_textures_th(tex, sio.optimize_dir)
+    for k, ref in zip(("dif", "nom", "spe", "rgh"), C.save_textures(tex[0].cpu().numpy())):
+        np.testing.assert_array_equal(io.read_png(sio.optimize_dir / f"{k}.png"), ref)
+    assert io.read_png(sio.optimize_dir / "tex.png").shape == (512, 512, 3)
+    sio.save_images_th(imgs, sio.rerender_dir)
+    saved = C.save_images(imgs.cpu().numpy())
+    for k, i in enumerate(idx):
+        np.testing.assert_array_equal(io.read_png(sio.rerender_dir / f"{i:02d}.png"), saved[k])
+    # imread / imwrite with the reference's flags and layouts
+    n = io.imread(ref_dir / "nom.png", "normal", (100, 80))
+    assert n.shape == (80, 100, 3)
+    np.testing.assert_array_equal(n.cpu().numpy(), C.imread_u8(maps["nom"], "normal", (100, 80)))
+    r = io.imread(ref_dir / "rgh.png", "rough")
+    assert r.shape == (300, 300)
+    io.imwrite(n, tmp_path / "n.png", "normal")
+    assert io.read_png(tmp_path / "n.png").shape == (80, 100, 3)
+
+
+def test_codec_argument_validation(pkg):
+    lib = pkg._lib.load()
+    assert lib.svbrdf_resize_lanczos4_u8(None, 4, 4, 3, None, 8, 8, None) == -1
+    t = torch.zeros(16, dtype=torch.uint8, device=DEV)
+    p = pkg._lib.ptr(t)
+    assert lib.svbrdf_resize_lanczos4_u8(p, 4, 4, 5, p, 8, 8, None) == -1
+    assert lib.svbrdf_resize_lanczos4_u8(p, 0, 4, 3, p, 8, 8, None) == -2
+    assert lib.svbrdf_images_from_u8(p, 1, 2, 2, 4, 1, p, None) == -1      # srgb needs 3 channels
+    assert lib.svbrdf_images_from_u8(p, 1, 2, 2, 3, 9, p, None) == -1      # unknown flag
+    assert lib.svbrdf_textures_from_u8(p, p, p, p, 2, 4, p, None) == -1
+    assert lib.svbrdf_textures_to_u8(p, 0, p, p, p, p, None) == -2
