@@ -55,56 +55,122 @@ struct AxisPlan {
     int w[8];
 };
 
-__device__ void lanczos4_plan(int d, int n_src, int n_dst, AxisPlan &p) {
+constexpr int kPlans = 2 * kTile;          // 16 column plans + 16 row plans per tile
+constexpr int kMaxSpan = 64;              // source rows a tile may stage in LDS
+
+// Source coordinate of destination index d: floor(f) and the fraction.
+__device__ __forceinline__ void source_coord(int d, int n_src, int n_dst, int &first, float &frac) {
     const double scale = 1.0 / ((double)n_dst / (double)n_src);
     const float f = (float)(((double)d + 0.5) * scale - 0.5);
     const float s = floorf(f);
-    const float x = f - s;
-    p.first = (int)s - 3;
-    const float xp3 = x + 3.0f;
-    const double a0 = -(double)xp3 * M_PI * 0.25;
-    const double s0 = sin(a0), c0 = cos(a0);
+    frac = f - s;
+    first = (int)s - 3;
+}
+
+// Float weight of tap i for fraction x, before normalisation.
+__device__ __forceinline__ float lanczos4_tap(float x, int i) {
     constexpr double kS45 = 0.70710678118654752440;
     // (-1)^(i+1) (cos(i pi/4), sin(i pi/4))
     constexpr double kA[8] = {-1.0, kS45, 0.0, -kS45, 1.0, -kS45, 0.0, kS45};
     constexpr double kB[8] = {0.0, kS45, -1.0, kS45, 0.0, -kS45, 1.0, -kS45};
-    float wf[8];
-    float total = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const float t = xp3 - (float)i;
-        if (fabsf(t) >= 1e-6f) {
-            const double y = -(double)t * M_PI * 0.25;
-            wf[i] = (float)((kA[i] * s0 + kB[i] * c0) / (y * y));
-        } else {
-            wf[i] = 1e30f;   // the unit impulse after normalisation
-        }
-        total = total + wf[i];
-    }
-    const float inv = 1.0f / total;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const float v = rintf((wf[i] * inv) * (float)(1 << kCoefBits));
-        p.w[i] = (int)fminf(fmaxf(v, -32768.0f), 32767.0f);
-    }
+    const float xp3 = x + 3.0f;
+    const float t = xp3 - (float)i;
+    if (fabsf(t) < 1e-6f) return 1e30f;   // the unit impulse after normalisation
+    const double a0 = -(double)xp3 * M_PI * 0.25;
+    const double y = -(double)t * M_PI * 0.25;
+    return (float)((kA[i] * sin(a0) + kB[i] * cos(a0)) / (y * y));
 }
 
+// The tile's 16 column and 16 row plans, one tap per thread (32 x 8 = 256):
+// float weights, then the sequential float sum of each plan's 8 weights and
+// its reciprocal, then the rounded 11-bit weights.
+__device__ void tile_plans(int x0, int y0, int sh, int sw, int dh, int dw, AxisPlan *plans, float (*wtmp)[8],
+                           float *winv) {
+    const int plan = threadIdx.x >> 3, i = threadIdx.x & 7;
+    const bool is_x = plan < kTile;
+    const int d = is_x ? x0 + plan : y0 + plan - kTile;
+    const bool live = d < (is_x ? dw : dh);
+    int first = 0;
+    float frac = 0.0f;
+    if (live) {
+        source_coord(d, is_x ? sw : sh, is_x ? dw : dh, first, frac);
+        wtmp[plan][i] = lanczos4_tap(frac, i);
+    }
+    __syncthreads();
+    if (live && i == 0) {
+        float total = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) total = total + wtmp[plan][k];
+        winv[plan] = 1.0f / total;
+        plans[plan].first = first;
+    }
+    __syncthreads();
+    if (live) {
+        const float v = rintf((wtmp[plan][i] * winv[plan]) * (float)(1 << kCoefBits));
+        plans[plan].w[i] = (int)fminf(fmaxf(v, -32768.0f), 32767.0f);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint8_t finish(int acc) {
+    const int v = (acc + (1 << (2 * kCoefBits - 1))) >> (2 * kCoefBits);
+    return (uint8_t)min(max(v, 0), 255);
+}
+
+// One 16 x 16 output tile per block.  When the tile's source rows fit
+// kMaxSpan, the horizontal 8-tap sums of those rows are staged in LDS and the
+// vertical pass reads them (separable: 8 + 8 taps per output instead of 64);
+// otherwise (strong downscaling) each output sums its 8 x 8 taps directly.
+// Integer sums make both orders exact and equal.
 template <int C>
 __global__ __launch_bounds__(kBlock) void resize_lanczos4_kernel(const uint8_t *__restrict__ src, int sh, int sw,
                                                                  uint8_t *__restrict__ dst, int dh, int dw) {
-    __shared__ AxisPlan plan_x[kTile], plan_y[kTile];
+    __shared__ AxisPlan plans[kPlans];
+    __shared__ float wtmp[kPlans][8];
+    __shared__ float winv[kPlans];
+    __shared__ int hsum[kMaxSpan][kTile][C];
     const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
-    const int t = threadIdx.x;
-    if (t < kTile) {
-        if (x0 + t < dw) lanczos4_plan(x0 + t, sw, dw, plan_x[t]);
-    } else if (t < 2 * kTile) {
-        if (y0 + t - kTile < dh) lanczos4_plan(y0 + t - kTile, sh, dh, plan_y[t - kTile]);
+    tile_plans(x0, y0, sh, sw, dh, dw, plans, wtmp, winv);
+    const AxisPlan *px = plans, *py = plans + kTile;
+    const int rows_live = min(kTile, dh - y0), cols_live = min(kTile, dw - x0);
+    const int r0 = py[0].first;
+    const int span = py[rows_live - 1].first + 8 - r0;   // uniform across the block
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    if (span <= kMaxSpan) {
+        for (int idx = threadIdx.x; idx < span * kTile; idx += kBlock) {
+            const int r = idx / kTile, xx = idx % kTile;
+            if (xx >= cols_live) continue;
+            const uint8_t *row = src + (size_t)min(max(r0 + r, 0), sh - 1) * sw * C;
+            const AxisPlan &ax = px[xx];
+            int h[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) h[c] = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint8_t *q = row + (size_t)min(max(ax.first + j, 0), sw - 1) * C;
+#pragma unroll
+                for (int c = 0; c < C; ++c) h[c] += (int)q[c] * ax.w[j];
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) hsum[r][xx][c] = h[c];
+        }
+        __syncthreads();
+        if (tx >= cols_live || ty >= rows_live) return;
+        const AxisPlan &ay = py[ty];
+        int acc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] += hsum[ay.first + k - r0][tx][c] * ay.w[k];
+        uint8_t *o = dst + ((size_t)(y0 + ty) * dw + x0 + tx) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) o[c] = finish(acc[c]);
+        return;
     }
-    __syncthreads();
-    const int tx = t % kTile, ty = t / kTile;
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= dw || y >= dh) return;
-    const AxisPlan &ax = plan_x[tx], &ay = plan_y[ty];
+    if (tx >= cols_live || ty >= rows_live) return;
+    const AxisPlan &ax = px[tx], &ay = py[ty];
     int cols[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) cols[j] = min(max(ax.first + j, 0), sw - 1);
@@ -113,8 +179,7 @@ __global__ __launch_bounds__(kBlock) void resize_lanczos4_kernel(const uint8_t *
     for (int c = 0; c < C; ++c) acc[c] = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int row = min(max(ay.first + k, 0), sh - 1);
-        const uint8_t *r = src + (size_t)row * sw * C;
+        const uint8_t *r = src + (size_t)min(max(ay.first + k, 0), sh - 1) * sw * C;
         int h[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) h[c] = 0;
@@ -125,12 +190,9 @@ __global__ __launch_bounds__(kBlock) void resize_lanczos4_kernel(const uint8_t *
 #pragma unroll
         for (int c = 0; c < C; ++c) acc[c] += h[c] * ay.w[k];
     }
-    uint8_t *o = dst + ((size_t)y * dw + x) * C;
+    uint8_t *o = dst + ((size_t)(y0 + ty) * dw + x0 + tx) * C;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-        const int v = (acc[c] + (1 << (2 * kCoefBits - 1))) >> (2 * kCoefBits);
-        o[c] = (uint8_t)min(max(v, 0), 255);
-    }
+    for (int c = 0; c < C; ++c) o[c] = finish(acc[c]);
 }
 
 // ---------------------------------------------------------------------------
