@@ -82,6 +82,30 @@ int svbrdf_render(const float *textures, int res, int n_views, float size,
                   const float *camera_pos, const float *light_pos,
                   const float *light_pow, float *images, void *stream);
 
+/* BRDF terms of the render (bit set).  SVBRDF_TERMS_EVAL is what
+ * Microfacet.eval uses: GGX (src/microfacet.py:28-32) and the
+ * spherical-Gaussian Schlick Fresnel (:43-45).  The reference also defines
+ * Beckmann (:34-38) and Fresnel_f0 (:40-41, here with f0 = the specular map)
+ * without calling them from eval. */
+enum svbrdf_terms {
+    SVBRDF_TERMS_EVAL = 0,
+    SVBRDF_NDF_BECKMANN = 1,       /* D = exp(-t^2/a^2) / (pi a^2 (n.h)^4), t^2 = (1 - (n.h)^2)/(n.h)^2 */
+    SVBRDF_FRESNEL_SCHLICK = 2,    /* F = spe + (1 - spe)(1 - v.h)^5                                    */
+};
+
+/* svbrdf_render / svbrdf_render_backward with a choice of terms (any
+ * combination of the bits above; SVBRDF_TERMS_EVAL gives exactly
+ * svbrdf_render / svbrdf_render_backward).  The fused step always uses the
+ * eval terms. */
+int svbrdf_render_terms(const float *textures, int res, int n_views, float size,
+                        const float *camera_pos, const float *light_pos,
+                        const float *light_pow, int terms, float *images, void *stream);
+int svbrdf_render_backward_terms(const float *textures, int res, int n_views, float size,
+                                 const float *camera_pos, const float *light_pos,
+                                 const float *light_pow, int terms, const float *grad_images,
+                                 float *grad_textures, float *grad_light_pow,
+                                 void *workspace, size_t workspace_bytes, void *stream);
+
 /* Adjoint of svbrdf_render: the backward of Microfacet.eval under torch
  * autograd (src/microfacet.py:64-120).  Given dL/dimages it writes
  * dL/dtextures (overwritten, not accumulated) and, when grad_light_pow is not

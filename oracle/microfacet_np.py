@@ -112,13 +112,21 @@ def decode(tex, dtype=np.float32):
                 nlen=nlen[0], n=n)
 
 
-def _terms(mp, geo, light_pow, dtype, dot_scale=None):
+NDFS = ("ggx", "beckmann")          # src/microfacet.py:28-32, :34-38
+FRESNELS = ("sg", "schlick")        # src/microfacet.py:43-45, :40-41
+
+
+def _terms(mp, geo, light_pow, dtype, dot_scale=None, ndf="ggx", fresnel="sg"):
     """Forward BRDF terms shared by render() and the adjoint.
 
     src/microfacet.py:96-117 with GGX :28-32, Fresnel :43-45, Smith :47-52.
-    ``dot_scale`` = (sv, sl, sh) multiplies n.v, n.l, n.h (used only by
-    :func:`fp32_noise_scale`).
+    ``ndf="beckmann"`` uses Beckmann (:34-38) instead of GGX, and
+    ``fresnel="schlick"`` uses Fresnel_f0(v.h, specular) (:40-41) instead of
+    Fresnel(v.h, specular): the reference defines both but eval calls GGX and
+    Fresnel.  ``dot_scale`` = (sv, sl, sh) multiplies n.v, n.l, n.h (used only
+    by :func:`fp32_noise_scale`).
     """
+    assert ndf in NDFS and fresnel in FRESNELS
     one, eps = _c(1, dtype), _c(EPS, dtype)
     pi = _c(np.pi, dtype)
     n = mp["n"][None]
@@ -137,15 +145,26 @@ def _terms(mp, geo, light_pow, dtype, dot_scale=None):
     dif, spe, rgh = mp["dif"][None], mp["spe"][None], mp["rgh"][None]
     f1 = dif / pi
     f1 = f1 * (one - spe)
-    # GGX(n_dot_h, roughness**2)
     alpha = rgh ** 2
     c2 = nh ** 2
     a2 = alpha ** 2
-    den = c2 * a2 + (one - c2)
-    qd = pi * den ** 2 + eps
-    D = a2 / qd
-    # Fresnel(v_dot_h, specular)
-    sphg = np.power(_c(2, dtype), ((_c(FRESNEL_A, dtype) * vh) + _c(FRESNEL_B, dtype)) * vh)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if ndf == "ggx":
+            # GGX(n_dot_h, roughness**2)
+            den = c2 * a2 + (one - c2)
+            qd = pi * den ** 2 + eps
+            D = a2 / qd
+        else:
+            # Beckmann(n_dot_h, roughness**2): exp(-t2/a2) / (pi a2 c2^2)
+            den = (one - c2) / c2                 # t2
+            qd = pi * a2 * c2 ** 2
+            D = np.exp(-den / a2) / qd
+    if fresnel == "sg":
+        # Fresnel(v_dot_h, specular)
+        sphg = np.power(_c(2, dtype), ((_c(FRESNEL_A, dtype) * vh) + _c(FRESNEL_B, dtype)) * vh)
+    else:
+        # Fresnel_f0(v_dot_h, specular): f0 + (1 - f0) (1 - cos)^5
+        sphg = (one - vh) ** 5
     F = spe + (one - spe) * sphg
     # Smith(n_dot_v, n_dot_l, roughness**2)
     k = alpha * _c(0.5, dtype) + eps
@@ -166,7 +185,7 @@ def _terms(mp, geo, light_pow, dtype, dot_scale=None):
 
 
 def render(tex, res, size, cam, light, light_pow, dtype=np.float32, return_all=False,
-           dot_scale=None, rows=None):
+           dot_scale=None, rows=None, ndf="ggx", fresnel="sg"):
     """Forward render, src/microfacet.py:84-120.  Returns ``[V,3,R,R]``
     (or ``[V,3,len(rows),R]`` for a slice ``rows`` of image rows).
 
@@ -178,7 +197,7 @@ def render(tex, res, size, cam, light, light_pow, dtype=np.float32, return_all=F
         tex = tex.reshape(9, res, res)[:, rows]
     mp = decode(tex, dtype)
     geo = geometry(res, size, cam, light, dtype, rows)
-    tm = _terms(mp, geo, light_pow, dtype, dot_scale)
+    tm = _terms(mp, geo, light_pow, dtype, dot_scale, ndf, fresnel)
     ic = np.clip(tm["img"], _c(EPS, dtype), _c(1, dtype))
     out = ic ** _c(1 / GAMMA, dtype)
     if return_all:
@@ -186,7 +205,7 @@ def render(tex, res, size, cam, light, light_pow, dtype=np.float32, return_all=F
     return out
 
 
-def fp32_noise_scale(tex, res, size, cam, light, light_pow, ulps=1.0, rows=None):
+def fp32_noise_scale(tex, res, size, cam, light, light_pow, ulps=1.0, rows=None, ndf="ggx", fresnel="sg"):
     """Per-element FP32 rounding sensitivity of :func:`render`, in FP64.
 
     The ill-conditioned intermediates of the renderer are the three dot
@@ -198,14 +217,14 @@ def fp32_noise_scale(tex, res, size, cam, light, light_pow, ulps=1.0, rows=None)
     (FP64, shape [V,3,R,R]).
     """
     u = ulps * 2.0 ** -24
-    base = render(tex, res, size, cam, light, light_pow, np.float64, rows=rows)
+    base = render(tex, res, size, cam, light, light_pow, np.float64, rows=rows, ndf=ndf, fresnel=fresnel)
     s = np.zeros_like(base)
     for k in range(3):
         for sign in (1.0, -1.0):
             sc = [1.0, 1.0, 1.0]
             sc[k] = 1.0 + sign * u
             d = np.abs(render(tex, res, size, cam, light, light_pow, np.float64,
-                              dot_scale=sc, rows=rows) - base)
+                              dot_scale=sc, rows=rows, ndf=ndf, fresnel=fresnel) - base)
             s_k = d if sign > 0 else np.maximum(s_k, d)
         s += s_k
     return s
@@ -243,7 +262,8 @@ def loss_grad(out, target, mode, dtype=np.float32):
     return _c(2, dtype) * r / N / out
 
 
-def render_vjp(tex, res, size, cam, light, light_pow, gout, dtype=np.float32, rows=None):
+def render_vjp(tex, res, size, cam, light, light_pow, gout, dtype=np.float32, rows=None, ndf="ggx",
+               fresnel="sg"):
     """Vector-Jacobian product of :func:`render`: returns (dtex [9,R,R], dP [3]).
 
     Hand-derived adjoint of src/microfacet.py:64-120 with torch's gradient
@@ -252,7 +272,8 @@ def render_vjp(tex, res, size, cam, light, light_pow, gout, dtype=np.float32, ro
     Jacobian (I - n n^T)/|n| even when |n| = 1, and expanded tensors sum
     their gradients over views and channels.  Positions carry no gradient.
     """
-    out, st = render(tex, res, size, cam, light, light_pow, dtype, return_all=True, rows=rows)
+    out, st = render(tex, res, size, cam, light, light_pow, dtype, return_all=True, rows=rows, ndf=ndf,
+                     fresnel=fresnel)
     mp, geo, tm = st["mp"], st["geo"], st["tm"]
     one, eps, zero = _c(1, dtype), _c(EPS, dtype), _c(0, dtype)
     pi = _c(np.pi, dtype)
@@ -288,9 +309,16 @@ def render_vjp(tex, res, size, cam, light, light_pow, gout, dtype=np.float32, ro
     g_k = g_G * (g1l * (-nv * (one - nv)) / (dv * dv) + g1v * (-nl * (one - nl)) / (dl * dl))
     # k = alpha/2 + eps ; D = a2 / (pi den^2 + eps), a2 = alpha^2, den = c2 a2 + 1 - c2
     alpha, a2, den, qd, c2, nh = tm["alpha"], tm["a2"], tm["den"], tm["qd"], tm["c2"], tm["nh"]
-    dD_dden = -a2 * _c(2, dtype) * pi * den / (qd * qd)
-    g_a2 = g_D * (one / qd + dD_dden * c2)
-    g_c2 = g_D * dD_dden * (a2 - one)
+    if ndf == "ggx":
+        dD_dden = -a2 * _c(2, dtype) * pi * den / (qd * qd)
+        g_a2 = g_D * (one / qd + dD_dden * c2)
+        g_c2 = g_D * dD_dden * (a2 - one)
+    else:
+        # D = exp(-t2/a2) / (pi a2 c2^2), t2 = (1 - c2)/c2 (den holds t2):
+        # dD/dc2 = D (1/(a2 c2^2) - 2/c2), dD/da2 = D (t2/a2^2 - 1/a2)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            g_c2 = g_D * D * (one / (a2 * c2 * c2) - _c(2, dtype) / c2)
+            g_a2 = g_D * D * (den / (a2 * a2) - one / a2)
     g_nh = g_c2 * _c(2, dtype) * nh
     g_alpha = g_k * _c(0.5, dtype) + g_a2 * _c(2, dtype) * alpha
     g_rgh = (g_alpha * _c(2, dtype) * mp["rgh"][None]).sum(0)   # [1,R,R]: summed over views

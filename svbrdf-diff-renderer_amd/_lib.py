@@ -21,6 +21,7 @@ EHIP = -4
 LOSS_L2 = 0
 LOSS_LOG_L2 = 1
 IMREAD_NONE, IMREAD_SRGB, IMREAD_ROUGH, IMREAD_NORMAL = 0, 1, 2, 3
+TERMS_EVAL, NDF_BECKMANN, FRESNEL_SCHLICK = 0, 1, 2
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
@@ -36,6 +37,10 @@ SIGNATURES = {
     "svbrdf_render_backward_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),
     "svbrdf_render_backward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp, _vp,
                                               _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "svbrdf_render_terms": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp, _vp,
+                                           ctypes.c_int, _vp, _vp]),
+    "svbrdf_render_backward_terms": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp, _vp,
+                                                    ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "svbrdf_prepare_targets": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "svbrdf_step_state_init": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp]),
     "svbrdf_fused_step_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),

@@ -35,6 +35,12 @@ constexpr float kInvGamma = 0.454545454545454545f;  // 1/2.2, microfacet.py:120
 constexpr float kLn2 = 0.693147180559945309f;
 constexpr float kFresA = -5.55473f;                 // microfacet.py:44
 constexpr float kFresB = -6.98316f;
+constexpr float kLog2E = 1.44269504088896340736f;
+
+// BRDF term selection (bit set).  0 = the terms Microfacet.eval uses: GGX
+// (microfacet.py:28-32) and the spherical-Gaussian Fresnel (:43-45).
+constexpr int kTermsBeckmann = 1;   // Beckmann NDF instead of GGX (:34-38)
+constexpr int kTermsSchlick = 2;    // Fresnel_f0(v.h, specular) = spe + (1-spe)(1-v.h)^5 (:40-41)
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -150,7 +156,7 @@ struct Shade {
     float nl_d2, f[3], img[3];
 };
 
-template <bool COLOC>
+template <bool COLOC, int TERMS = 0>
 __device__ __forceinline__ void shade(const Maps &m, const Geom &g, const float P[3], Shade &s) {
     s.nv_raw = m.n[0] * g.v[0] + m.n[1] * g.v[1] + m.n[2] * g.v[2];
     if (COLOC) {
@@ -171,11 +177,26 @@ __device__ __forceinline__ void shade(const Maps &m, const Geom &g, const float 
     const float xy = m.n[2] * g.h[0] - m.n[0] * g.h[2];
     const float xz = m.n[0] * g.h[1] - m.n[1] * g.h[0];
     const float one_m_c2 = s.nh_raw < 0.0f ? 1.0f : xx * xx + xy * xy + xz * xz;
-    s.den = s.c2 * m.a2 + one_m_c2;
-    s.iqd = frcp(kPi * (s.den * s.den) + kEps);
-    s.D = m.a2 * s.iqd;
-    // Fresnel(v.h, spe)
-    const float sphg = fexp2((kFresA * g.vh + kFresB) * g.vh);
+    if constexpr (TERMS & kTermsBeckmann) {
+        // Beckmann: exp(-t2/a2) / (pi a2 c2^2), t2 = (1 - c2)/c2 (den holds t2,
+        // iqd the reciprocal of the denominator)
+        s.den = one_m_c2 * frcp(s.c2);
+        s.iqd = frcp(kPi * m.a2 * (s.c2 * s.c2));
+        s.D = fexp2(-s.den * frcp(m.a2) * kLog2E) * s.iqd;
+    } else {
+        s.den = s.c2 * m.a2 + one_m_c2;
+        s.iqd = frcp(kPi * (s.den * s.den) + kEps);
+        s.D = m.a2 * s.iqd;
+    }
+    // Fresnel(v.h, spe): spe + (1 - spe) sphg
+    float sphg;
+    if constexpr (TERMS & kTermsSchlick) {
+        const float o = 1.0f - g.vh;
+        const float o2 = o * o;
+        sphg = o2 * o2 * o;
+    } else {
+        sphg = fexp2((kFresA * g.vh + kFresB) * g.vh);
+    }
     s.omsphg = 1.0f - sphg;
     // Smith(n.v, n.l, rgh^2)
     s.idv = frcp(s.nv * m.omk + m.k);
@@ -216,7 +237,7 @@ __device__ __forceinline__ void accum_zero(Accum &a) {
 
 // Backward of shade() for one view given dL/dimg_c (already masked by the
 // output clamp).  Accumulates into `a`.
-template <bool LIGHT_GRAD>
+template <bool LIGHT_GRAD, int TERMS = 0>
 __device__ __forceinline__ void shade_backward(const Maps &m, const Geom &g, const Shade &s,
                                                const float P[3], const float gimg[3], Accum &a) {
     float W = 0.f, gnl = 0.f;
@@ -241,10 +262,19 @@ __device__ __forceinline__ void shade_backward(const Maps &m, const Geom &g, con
     gnl += gq * 4.0f * s.nv + gG * s.g1v * dg1l_dc;
     const float gk = -gG * (s.g1l * s.nv * (1.0f - s.nv) * s.idv * s.idv +
                             s.g1v * s.nl * (1.0f - s.nl) * s.idl * s.idl);
-    // D = a2 / (pi den^2 + eps), den = c2 a2 + 1 - c2
-    const float dD_dden = -2.0f * kPi * m.a2 * s.den * s.iqd * s.iqd;
-    const float ga2 = gD * (s.iqd + dD_dden * s.c2);
-    const float gc2 = gD * dD_dden * (m.a2 - 1.0f);
+    float ga2, gc2;
+    if constexpr (TERMS & kTermsBeckmann) {
+        // D = exp(-t2/a2) / (pi a2 c2^2), t2 = (1 - c2)/c2:
+        // dD/dc2 = D (1/(a2 c2^2) - 2/c2), dD/da2 = D (t2/a2^2 - 1/a2)
+        const float ic2 = frcp(s.c2), ia2 = frcp(m.a2);
+        gc2 = gD * s.D * (ia2 * ic2 * ic2 - 2.0f * ic2);
+        ga2 = gD * s.D * (s.den * ia2 * ia2 - ia2);
+    } else {
+        // D = a2 / (pi den^2 + eps), den = c2 a2 + 1 - c2
+        const float dD_dden = -2.0f * kPi * m.a2 * s.den * s.iqd * s.iqd;
+        ga2 = gD * (s.iqd + dD_dden * s.c2);
+        gc2 = gD * dD_dden * (m.a2 - 1.0f);
+    }
     float gnh = gc2 * 2.0f * s.nh;
     // alpha = rgh^2 feeds k = alpha/2 + eps and a2 = alpha^2
     a.A += gk * 0.5f + ga2 * 2.0f * m.alpha;

@@ -155,7 +155,7 @@ __device__ __forceinline__ double block_sum_d(double x, double *lds) {
 // ---------------------------------------------------------------------------
 // Forward render: NPIX pixels per thread (block-strided, so each plane access
 // of a wave stays one contiguous 256-byte run).
-template <bool COLOC, int NPIX>
+template <bool COLOC, int NPIX, int TERMS = 0>
 __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float *__restrict__ tex, unsigned npix,
                                               int n_views, const float *__restrict__ cam,
                                               const float *__restrict__ light, const float P[3],
@@ -184,7 +184,7 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
                 Geom g;
                 geometry<false>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
                 Shade s;
-                shade<false>(m[q], g, P, s);
+                shade<false, TERMS>(m[q], g, P, s);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) img[c] = s.img[c];
             }
@@ -199,7 +199,9 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
     }
 }
 
-template <int NPIX>
+// TERMS != 0 (alternate BRDF terms) always takes the general path: the
+// co-located specialisation hard-codes GGX and the spherical-Gaussian Fresnel.
+template <int NPIX, int TERMS = 0>
 __global__ __launch_bounds__(kBlock) void render_kernel(
     const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, const float *__restrict__ lpow, float *__restrict__ out) {
@@ -209,15 +211,15 @@ __global__ __launch_bounds__(kBlock) void render_kernel(
     for (int q = 0; q < NPIX; ++q)
         px[q] = pixel_at(blockIdx.x * (kBlock * NPIX) + q * kBlock + threadIdx.x, npix, res, size);
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
-    if (all_colocated(cam, light, n_views))
+    if (TERMS == 0 && all_colocated(cam, light, n_views))
         render_pixels<true, NPIX>(px, tex, npix, n_views, cam, light, P, out);
     else
-        render_pixels<false, NPIX>(px, tex, npix, n_views, cam, light, P, out);
+        render_pixels<false, NPIX, TERMS>(px, tex, npix, n_views, cam, light, P, out);
 }
 
 // ---------------------------------------------------------------------------
 // Adjoint of the render for an arbitrary dL/dimages.
-template <bool COLOC, bool LIGHT_GRAD>
+template <bool COLOC, bool LIGHT_GRAD, int TERMS = 0>
 __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__restrict__ tex, unsigned npix,
                                                int n_views, const float *__restrict__ cam,
                                                const float *__restrict__ light, const float P[3],
@@ -257,11 +259,11 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
             Geom g;
             geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
             Shade s;
-            shade<false>(m, g, P, s);
+            shade<false, TERMS>(m, g, P, s);
             float gimg[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) gimg[c] = dimg(s.img[c], go[c]);
-            shade_backward<LIGHT_GRAD>(m, g, s, P, gimg, a);
+            shade_backward<LIGHT_GRAD, TERMS>(m, g, s, P, gimg, a);
         }
     }
     if constexpr (COLOC) coloc_finish(a);
@@ -273,7 +275,7 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
     }
 }
 
-template <bool LIGHT_GRAD>
+template <bool LIGHT_GRAD, int TERMS = 0>
 __global__ __launch_bounds__(kBlock) void backward_kernel(
     const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, const float *__restrict__ lpow, const float *__restrict__ gout,
@@ -284,10 +286,10 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
     Accum a;
     accum_zero(a);
-    if (all_colocated(cam, light, n_views))
+    if (TERMS == 0 && all_colocated(cam, light, n_views))
         backward_pixel<true, LIGHT_GRAD>(px, tex, npix, n_views, cam, light, P, gout, gtex, a);
     else
-        backward_pixel<false, LIGHT_GRAD>(px, tex, npix, n_views, cam, light, P, gout, gtex, a);
+        backward_pixel<false, LIGHT_GRAD, TERMS>(px, tex, npix, n_views, cam, light, P, gout, gtex, a);
     if (LIGHT_GRAD) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -922,14 +924,31 @@ size_t svbrdf_step_state_bytes(void) { return sizeof(svbrdf_step_state); }
 int svbrdf_max_res(void) { return kMaxRes; }
 int svbrdf_max_views(void) { return kMaxViews; }
 
-int svbrdf_render(const float *textures, int res, int n_views, float size, const float *camera_pos,
-                  const float *light_pos, const float *light_pow, float *images, void *stream) {
+int svbrdf_render_terms(const float *textures, int res, int n_views, float size, const float *camera_pos,
+                        const float *light_pos, const float *light_pow, int terms, float *images, void *stream) {
     int rc = check_common("svbrdf_render", textures, res, n_views, camera_pos, light_pos, light_pow);
     if (rc) return rc;
     if (!images) return fail(SVBRDF_EINVAL, "svbrdf_render: null images");
-    hipLaunchKernelGGL(render_kernel<1>, dim3(n_blocks(res)), dim3(kBlock), 0, (hipStream_t)stream, textures,
-                       res, n_views, size, camera_pos, light_pos, light_pow, images);
+    if (terms < 0 || terms > (SVBRDF_NDF_BECKMANN | SVBRDF_FRESNEL_SCHLICK))
+        return fail(SVBRDF_EINVAL, "svbrdf_render: unknown terms %d", terms);
+    hipStream_t s = (hipStream_t)stream;
+#define SVBRDF_LAUNCH_RENDER(T)                                                                               \
+    hipLaunchKernelGGL((render_kernel<1, T>), dim3(n_blocks(res)), dim3(kBlock), 0, s, textures, res, n_views, \
+                       size, camera_pos, light_pos, light_pow, images)
+    switch (terms) {
+        case 0: SVBRDF_LAUNCH_RENDER(0); break;
+        case 1: SVBRDF_LAUNCH_RENDER(1); break;
+        case 2: SVBRDF_LAUNCH_RENDER(2); break;
+        default: SVBRDF_LAUNCH_RENDER(3); break;
+    }
+#undef SVBRDF_LAUNCH_RENDER
     return check_launch("svbrdf_render");
+}
+
+int svbrdf_render(const float *textures, int res, int n_views, float size, const float *camera_pos,
+                  const float *light_pos, const float *light_pow, float *images, void *stream) {
+    return svbrdf_render_terms(textures, res, n_views, size, camera_pos, light_pos, light_pow, SVBRDF_TERMS_EVAL,
+                               images, stream);
 }
 
 size_t svbrdf_render_backward_workspace_bytes(int res) {
@@ -937,14 +956,16 @@ size_t svbrdf_render_backward_workspace_bytes(int res) {
     return (size_t)n_blocks(res) * 4 * sizeof(float);
 }
 
-int svbrdf_render_backward(const float *textures, int res, int n_views, float size, const float *camera_pos,
-                           const float *light_pos, const float *light_pow, const float *grad_images,
-                           float *grad_textures, float *grad_light_pow, void *workspace,
-                           size_t workspace_bytes, void *stream) {
+int svbrdf_render_backward_terms(const float *textures, int res, int n_views, float size, const float *camera_pos,
+                                 const float *light_pos, const float *light_pow, int terms,
+                                 const float *grad_images, float *grad_textures, float *grad_light_pow,
+                                 void *workspace, size_t workspace_bytes, void *stream) {
     int rc = check_common("svbrdf_render_backward", textures, res, n_views, camera_pos, light_pos, light_pow);
     if (rc) return rc;
     if (!grad_images || !grad_textures)
         return fail(SVBRDF_EINVAL, "svbrdf_render_backward: null gradient pointer");
+    if (terms < 0 || terms > (SVBRDF_NDF_BECKMANN | SVBRDF_FRESNEL_SCHLICK))
+        return fail(SVBRDF_EINVAL, "svbrdf_render_backward: unknown terms %d", terms);
     hipStream_t s = (hipStream_t)stream;
     const unsigned nb = n_blocks(res);
     if (grad_light_pow) {
@@ -952,16 +973,39 @@ int svbrdf_render_backward(const float *textures, int res, int n_views, float si
         if (!workspace || workspace_bytes < need)
             return fail(SVBRDF_ESPACE, "svbrdf_render_backward: workspace of %zu bytes < %zu needed",
                         workspace_bytes, need);
-        hipLaunchKernelGGL(backward_kernel<true>, dim3(nb), dim3(kBlock), 0, s, textures, res, n_views, size,
-                           camera_pos, light_pos, light_pow, grad_images, grad_textures, (float *)workspace);
-        if ((rc = check_launch("svbrdf_render_backward"))) return rc;
+    }
+    float *parts = grad_light_pow ? (float *)workspace : nullptr;
+#define SVBRDF_LAUNCH_BWD(L, T)                                                                                \
+    hipLaunchKernelGGL((backward_kernel<L, T>), dim3(nb), dim3(kBlock), 0, s, textures, res, n_views, size,      \
+                       camera_pos, light_pos, light_pow, grad_images, grad_textures, parts)
+#define SVBRDF_LAUNCH_BWD_T(L)                                                                                 \
+    switch (terms) {                                                                                            \
+        case 0: SVBRDF_LAUNCH_BWD(L, 0); break;                                                                 \
+        case 1: SVBRDF_LAUNCH_BWD(L, 1); break;                                                                 \
+        case 2: SVBRDF_LAUNCH_BWD(L, 2); break;                                                                 \
+        default: SVBRDF_LAUNCH_BWD(L, 3); break;                                                                \
+    }
+    if (grad_light_pow) {
+        SVBRDF_LAUNCH_BWD_T(true);
+    } else {
+        SVBRDF_LAUNCH_BWD_T(false);
+    }
+#undef SVBRDF_LAUNCH_BWD_T
+#undef SVBRDF_LAUNCH_BWD
+    if ((rc = check_launch("svbrdf_render_backward"))) return rc;
+    if (grad_light_pow)
         hipLaunchKernelGGL(reduce_light_kernel, dim3(1), dim3(kBlock), 0, s, (const float *)workspace, nb,
                            grad_light_pow);
-    } else {
-        hipLaunchKernelGGL(backward_kernel<false>, dim3(nb), dim3(kBlock), 0, s, textures, res, n_views, size,
-                           camera_pos, light_pos, light_pow, grad_images, grad_textures, (float *)nullptr);
-    }
     return check_launch("svbrdf_render_backward");
+}
+
+int svbrdf_render_backward(const float *textures, int res, int n_views, float size, const float *camera_pos,
+                           const float *light_pos, const float *light_pow, const float *grad_images,
+                           float *grad_textures, float *grad_light_pow, void *workspace,
+                           size_t workspace_bytes, void *stream) {
+    return svbrdf_render_backward_terms(textures, res, n_views, size, camera_pos, light_pos, light_pow,
+                                        SVBRDF_TERMS_EVAL, grad_images, grad_textures, grad_light_pow, workspace,
+                                        workspace_bytes, stream);
 }
 
 int svbrdf_prepare_targets(const float *targets, size_t count, int loss_mode, float *prepared, void *stream) {

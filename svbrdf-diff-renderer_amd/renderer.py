@@ -6,7 +6,16 @@ the attributes its callers read (``res``, ``n_of_imgs``, ``eps``, ``f0``)
 and the two methods they call (``eval``, ``update_light``).  ``eval`` is
 differentiable with respect to the textures and the light power through
 :class:`RenderFunction`, whose backward is the hand-written adjoint kernel.
+
+Keyword-only ``ndf="beckmann"`` / ``fresnel="schlick"`` render with the
+reference's unused terms (Beckmann, microfacet.py:34-38; Fresnel_f0 with f0 =
+the specular map, :40-41) in both kernels; the defaults are eval's.  The
+public term methods of the reference class (GGX, Beckmann, Fresnel_f0,
+Fresnel, Smith, dot, normalize, get_dir, reconstruct_normal, tex2map) are kept
+as the same elementwise formulas on torch tensors for callers that use them
+directly; eval does not go through them.
 """
+import numpy as np
 import torch
 
 from . import _lib
@@ -38,8 +47,9 @@ class RenderFunction(torch.autograd.Function):
         lp = light_pow.detach().to(torch.float32).contiguous()
         res, V = renderer.res, renderer.n_of_imgs
         out = torch.empty((V, 3, res, res), dtype=torch.float32, device=tex.device)
-        _lib.call("svbrdf_render", _lib.ptr(tex), res, V, renderer.size, _lib.ptr(renderer.camera_pos),
-                  _lib.ptr(renderer.light_pos), _lib.ptr(lp), _lib.ptr(out), _lib.stream_handle(tex.device))
+        _lib.call("svbrdf_render_terms", _lib.ptr(tex), res, V, renderer.size, _lib.ptr(renderer.camera_pos),
+                  _lib.ptr(renderer.light_pos), _lib.ptr(lp), renderer.terms, _lib.ptr(out),
+                  _lib.stream_handle(tex.device))
         ctx.save_for_backward(tex, lp)
         ctx.renderer = renderer
         ctx.tex_shape = textures.shape
@@ -57,9 +67,10 @@ class RenderFunction(torch.autograd.Function):
             glp = torch.empty(3, dtype=torch.float32, device=tex.device)
             nbytes = _lib.query("svbrdf_render_backward_workspace_bytes", r.res)
             ws = torch.empty(nbytes // 4, dtype=torch.float32, device=tex.device)
-        _lib.call("svbrdf_render_backward", _lib.ptr(tex), r.res, r.n_of_imgs, r.size, _lib.ptr(r.camera_pos),
-                  _lib.ptr(r.light_pos), _lib.ptr(lp), _lib.ptr(go), _lib.ptr(gtex), _lib.ptr(glp),
-                  _lib.ptr(ws), 0 if ws is None else ws.numel() * 4, _lib.stream_handle(tex.device))
+        _lib.call("svbrdf_render_backward_terms", _lib.ptr(tex), r.res, r.n_of_imgs, r.size,
+                  _lib.ptr(r.camera_pos), _lib.ptr(r.light_pos), _lib.ptr(lp), r.terms, _lib.ptr(go),
+                  _lib.ptr(gtex), _lib.ptr(glp), _lib.ptr(ws), 0 if ws is None else ws.numel() * 4,
+                  _lib.stream_handle(tex.device))
         gtex = gtex.view(ctx.tex_shape) if ctx.needs_input_grad[0] else None
         if glp is not None:
             glp = glp.to(ctx.saved_tensors[1].dtype)
@@ -73,7 +84,14 @@ class Microfacet:
     power, as built by SvbrdfIO.load_calibration_th (src/svbrdf.py:134-148).
     """
 
-    def __init__(self, res, n, size, cl, device):
+    NDFS = {"ggx": 0, "beckmann": _lib.NDF_BECKMANN}
+    FRESNELS = {"sg": 0, "schlick": _lib.FRESNEL_SCHLICK}
+
+    def __init__(self, res, n, size, cl, device, *, ndf="ggx", fresnel="sg"):
+        if ndf not in self.NDFS or fresnel not in self.FRESNELS:
+            raise ValueError(f"ndf must be one of {list(self.NDFS)}, fresnel one of {list(self.FRESNELS)}")
+        self.ndf, self.fresnel = ndf, fresnel
+        self.terms = self.NDFS[ndf] | self.FRESNELS[fresnel]
         self.res = int(res)
         self.n_of_imgs = int(n)
         self.f0 = 0.04
@@ -107,3 +125,68 @@ class Microfacet:
         if textures.dim() != 4 or textures.shape[0] != 1 or textures.shape[1] != 9:
             raise ValueError(f"textures must be [1, 9, R, R], got {tuple(textures.shape)}")
         return RenderFunction.apply(textures, self.light_pow.reshape(3), self)
+
+    # ------------------------------------------------------------------
+    # The reference's public term methods (src/microfacet.py:28-79), as the
+    # same elementwise formulas on torch tensors.
+    def GGX(self, cos_h, alpha):
+        c2 = cos_h ** 2
+        a2 = alpha ** 2
+        den = c2 * a2 + (1 - c2)
+        return a2 / (np.pi * den ** 2 + self.eps)
+
+    def Beckmann(self, cos_h, alpha):
+        c2 = cos_h ** 2
+        t2 = (1 - c2) / c2
+        a2 = alpha ** 2
+        return torch.exp(-t2 / a2) / (np.pi * a2 * c2 ** 2)
+
+    def Fresnel_f0(self, cos, f0):
+        return f0 + (1 - f0) * (1 - cos) ** 5
+
+    def Fresnel(self, cos, specular):
+        sphg = torch.pow(2.0, ((-5.55473 * cos) - 6.98316) * cos)
+        return specular + (1.0 - specular) * sphg
+
+    def Smith(self, n_dot_v, n_dot_l, alpha):
+        k = alpha * 0.5 + self.eps
+        return (n_dot_v / (n_dot_v * (1.0 - k) + k)) * (n_dot_l / (n_dot_l * (1.0 - k) + k))
+
+    def dot(self, a, b):
+        return (a * b).sum(1, keepdim=True).expand(-1, 3, -1, -1)
+
+    def normalize(self, vec):
+        return vec / (vec.norm(2.0, 1, keepdim=True))
+
+    @property
+    def pos(self):
+        """[1, 3, R, R] pixel centres (x, -y, 0) in cm (src/microfacet.py:16-19)."""
+        c = ((torch.arange(self.res, dtype=torch.float32, device=self.device) + 0.5) / self.res - 0.5) * self.size
+        yy, xx = torch.meshgrid(c, c, indexing="ij")
+        return torch.stack((xx, -yy, torch.zeros_like(xx)))[None]
+
+    def get_dir(self, pos):
+        """src/microfacet.py:60-62: pos [n,3,R,R] (or [n,3]) -> unit direction
+        [n,3,R,R] and squared distance (expanded to 3 channels)."""
+        if pos.dim() == 2:
+            pos = pos.reshape(-1, 3, 1, 1)
+        vec = pos - self.pos
+        return self.normalize(vec), self.dot(vec, vec)
+
+    def reconstruct_normal(self, texture):
+        """src/microfacet.py:64-70 on [N, 2, R, R]."""
+        normal_x = texture[:, 0, :, :].clamp(-1, 1)
+        normal_y = texture[:, 1, :, :].clamp(-1, 1)
+        normal_xy = (normal_x ** 2 + normal_y ** 2).clamp(0, 1 - self.eps)
+        normal_z = (1 - normal_xy).sqrt()
+        return self.normalize(torch.stack((normal_x, normal_y, normal_z), 1))
+
+    def tex2map(self, textures):
+        """src/microfacet.py:72-79: textures [1,9,R,R] -> (normal, diffuse,
+        specular, roughness), each expanded to [n,3,R,R]."""
+        n = self.n_of_imgs
+        diffuse = (((textures[:, 0:3] + 1) / 2) ** 2.2).expand(n, -1, -1, -1)
+        normal = self.reconstruct_normal(textures[:, 3:5]).expand(n, -1, -1, -1)
+        roughness = (((textures[:, 5] + 1) / 2) ** 2.2).expand(n, 3, -1, -1)
+        specular = (((textures[:, 6:9] + 1) / 2) ** 2.2).expand(n, -1, -1, -1)
+        return normal, diffuse, specular, roughness

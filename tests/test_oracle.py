@@ -283,3 +283,64 @@ def test_fused_step_oracle_matches_reference_loop(golden_dir, loss, light):
         assert np.abs(t - ref).max() <= 2 * float(z["lr"]) * (k + 1)
         if light:
             np.testing.assert_allclose(P, z[f"{key}_light_pow"][k], rtol=1e-6)
+
+
+# ------------------------------------------ alternate terms (N4): Beckmann, Schlick
+def _single_pixel_terms(theta, rgh, spe, vh=None, ndf="ggx", fresnel="sg"):
+    """_terms for one pixel with n = +z and a half vector at each angle theta
+    (one 'view' per angle, v = l = h)."""
+    V = len(theta)
+    h = np.stack([np.sin(theta), np.zeros(V), np.cos(theta)], 1)[:, :, None, None]
+    geo = dict(v=h, l=h, h=h, vh=np.ones((V, 1, 1, 1)) if vh is None else vh.reshape(V, 1, 1, 1),
+               d2=np.ones((V, 1, 1, 1)))
+    mp = dict(n=np.array([0.0, 0.0, 1.0])[:, None, None], dif=np.zeros((3, 1, 1)),
+              spe=np.full((3, 1, 1), spe), rgh=np.full((1, 1, 1), rgh))
+    return M._terms(mp, geo, np.ones(3), np.float64, ndf=ndf, fresnel=fresnel)
+
+
+@pytest.mark.parametrize("ndf", ["ggx", "beckmann"])
+@pytest.mark.parametrize("rgh", [0.4, 0.6, 0.8])
+def test_ndf_projected_area_normalisation(ndf, rgh):
+    # int D(h) (n.h) dw over the hemisphere = 1 for both microfacet distributions
+    theta = np.linspace(0, np.pi / 2, 40001)[:-1]
+    D = _single_pixel_terms(theta, rgh, 0.04, ndf=ndf)["D"][:, 0, 0, 0]
+    integrand = D * np.cos(theta) * np.sin(theta) * 2 * np.pi
+    total = np.sum((integrand[1:] + integrand[:-1]) / 2 * np.diff(theta))
+    assert abs(total - 1) < 2e-3, total
+
+
+def test_beckmann_and_schlick_closed_forms():
+    alpha = 0.5 ** 2
+    tm = _single_pixel_terms(np.array([0.0, np.arctan(0.3)]), 0.5, 0.04, ndf="beckmann")
+    # theta = 0: D = 1/(pi a2); tan^2 = 0.09: D = exp(-0.09/a2)/(pi a2 cos^4)
+    np.testing.assert_allclose(tm["D"][0, 0, 0, 0], 1 / (np.pi * alpha ** 2), rtol=1e-12)
+    c2 = np.cos(np.arctan(0.3)) ** 2
+    np.testing.assert_allclose(tm["D"][1, 0, 0, 0], np.exp(-0.09 / alpha ** 2) / (np.pi * alpha ** 2 * c2 ** 2),
+                               rtol=1e-12)
+    vh = np.array([1.0, 0.0, 0.5])
+    tm = _single_pixel_terms(np.zeros(3), 0.5, 0.04, vh=vh, fresnel="schlick")
+    np.testing.assert_allclose(tm["F"][:, 0, 0, 0], [0.04, 1.0, 0.04 + 0.96 / 32], rtol=1e-12)
+
+
+@pytest.mark.parametrize("ndf,fresnel", [("beckmann", "sg"), ("ggx", "schlick"), ("beckmann", "schlick")])
+def test_variant_gradients_match_central_differences(ndf, fresnel):
+    rng = np.random.default_rng(13)
+    res = 5
+    tex = rng.uniform(-0.8, 0.8, (9, res, res))
+    tex[3:5] = rng.uniform(-0.4, 0.4, (2, res, res))
+    cam = np.array([[-2.0, 1.0, 10.0], [3.0, -2.0, 11.0]])
+    light = np.array([[1.0, 2.0, 9.0], [-3.0, 1.5, 12.0]])
+    P = np.array([400.0, 500.0, 600.0])
+    gout = rng.normal(0, 1, (2, 3, res, res))
+
+    def L(t):
+        return float(np.sum(M.render(t, res, 6.0, cam, light, P, np.float64, ndf=ndf, fresnel=fresnel) * gout))
+
+    g, _ = M.render_vjp(tex, res, 6.0, cam, light, P, gout, np.float64, ndf=ndf, fresnel=fresnel)
+    h = 1e-6
+    for c, i, j in [(c, i, j) for c in range(9) for (i, j) in [(0, 1), (3, 4), (2, 2)]]:
+        tp, tm = tex.copy(), tex.copy()
+        tp[c, i, j] += h
+        tm[c, i, j] -= h
+        fd = (L(tp) - L(tm)) / (2 * h)
+        assert abs(fd - g[c, i, j]) <= 1e-6 * max(1.0, abs(fd)), (ndf, fresnel, c, i, j, fd, g[c, i, j])
