@@ -53,6 +53,14 @@ inline unsigned n_blocks(int res, int npix_per_thread = 1) {
 }
 
 // Geometry of view k, read with wave-uniform addresses (scalar loads).
+// Camera / light positions and light power are read-only for every kernel
+// that reads them and wave-uniform: read through the constant address space,
+// so they come in by scalar loads even when the pointer itself was loaded
+// from memory (the batched kernel's material table), where the generic
+// pointer would otherwise give a vector flat load per view.
+typedef const __attribute__((address_space(4))) float const_float;
+__device__ __forceinline__ float ldu(const float *p, int i) { return ((const_float *)p)[i]; }
+
 struct ViewPos {
     float cx, cy, cz, lx, ly, lz;
 };
@@ -60,8 +68,8 @@ struct ViewPos {
 __device__ __forceinline__ ViewPos load_view(const float *__restrict__ cam, const float *__restrict__ light,
                                              int k) {
     ViewPos p;
-    p.cx = cam[3 * k + 0]; p.cy = cam[3 * k + 1]; p.cz = cam[3 * k + 2];
-    p.lx = light[3 * k + 0]; p.ly = light[3 * k + 1]; p.lz = light[3 * k + 2];
+    p.cx = ldu(cam, 3 * k + 0); p.cy = ldu(cam, 3 * k + 1); p.cz = ldu(cam, 3 * k + 2);
+    p.lx = ldu(light, 3 * k + 0); p.ly = ldu(light, 3 * k + 1); p.lz = ldu(light, 3 * k + 2);
     return p;
 }
 
@@ -70,7 +78,7 @@ __device__ __forceinline__ ViewPos load_view(const float *__restrict__ cam, cons
 __device__ __forceinline__ bool all_colocated(const float *__restrict__ cam, const float *__restrict__ light,
                                               int n_views) {
     bool all = true;
-    for (int e = 0; e < 3 * n_views; ++e) all = all && (cam[e] == light[e]);
+    for (int e = 0; e < 3 * n_views; ++e) all = all && (ldu(cam, e) == ldu(light, e));
     return all;
 }
 
@@ -440,7 +448,7 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
     }
 #pragma unroll UNROLL
     for (int k = 0; k < n_views; ++k) {
-        const float cx = cam[3 * k + 0], cy = cam[3 * k + 1], cz = cam[3 * k + 2];
+        const float cx = ldu(cam, 3 * k + 0), cy = ldu(cam, 3 * k + 1), cz = ldu(cam, 3 * k + 2);
         float y[3];
         if (SVBRDF_TARGET_PREFETCH) {
             // issue view k+1's loads now (the last view re-reads its own, an L1/L2 hit)
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
         const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
         const unsigned first = (id - mt.tile_begin) * (unsigned)tile_pixels;
         const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
-        const float P[3] = {mt.light_pow[0], mt.light_pow[1], mt.light_pow[2]};
+        const float P[3] = {ldu(mt.light_pow, 0), ldu(mt.light_pow, 1), ldu(mt.light_pow, 2)};
         const bool coloc = all_colocated(mt.camera_pos, mt.light_pos, mt.n_views);
         float loss = 0.f;
         float gPs[3] = {0.f, 0.f, 0.f};
