@@ -248,7 +248,7 @@ def batch_leg(pkg, dev, steps=20, warmup=3):
             "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes,
             "autotuned": cfg, "candidates_timed": len(timings),
             "per_material_loop": {"ms_per_step": round(ms_loop, 4), "what": "FusedStep per material, 64 steps "
-                                  "(128 launches) in one HIP graph"},
+                                  "(one launch each) in one HIP graph"},
             "speedup_vs_loop": round(ms_loop / ms, 3)}
 
 
