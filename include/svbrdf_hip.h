@@ -69,8 +69,8 @@ int svbrdf_abi_version(void);
 const char *svbrdf_last_error(void);
 size_t svbrdf_step_state_bytes(void);
 
-/* Largest resolution and view count accepted (8192: the 9 planes of a map
- * must fit one 32-bit buffer range). */
+/* Largest resolution and view count accepted (7680: the 9 planes of a map
+ * must fit one buffer range below 2^31 bytes). */
 int svbrdf_max_res(void);
 int svbrdf_max_views(void);
 

@@ -37,7 +37,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTile = 16;                 // resize: 16 x 16 output pixels per block
 constexpr int kMaxSide = 16384;           // largest 8-bit image side accepted
-constexpr int kMaxRes = 8192;             // as the renderer's maximum resolution
+constexpr int kMaxRes = 7680;             // as the renderer's maximum resolution
 constexpr int kCoefBits = 11;             // fixed-point bits of the resize weights
 
 // ---------------------------------------------------------------------------

@@ -31,7 +31,7 @@ using namespace svbrdf;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxRes = 8192;   // 9 planes of res^2 floats within a 32-bit buffer range
+constexpr int kMaxRes = 7680;   // 9 planes of res^2 floats (and every plane offset) below 2^31 bytes
 constexpr int kMaxViews = 1024;
 
 using svbrdf_internal::check_launch;
@@ -100,7 +100,8 @@ struct Pix {
 // scalar registers within budget (27 per-plane descriptors spilled to vector
 // lanes).  Word 3 = 0x00020000 selects the 32-bit data format that untyped
 // gfx9 buffer accesses need.  All planes of a tensor must fit the 32-bit
-// range: 9 * res^2 * 4 bytes, hence kMaxRes = 8192.
+// range: 9 * res^2 * 4 bytes < 2^31, hence kMaxRes = 7680 (offsets and sizes
+// then fit a signed 32-bit value as well).
 struct Tensor {
     __amdgpu_buffer_rsrc_t r;
     unsigned stride;   // bytes per plane

@@ -130,3 +130,15 @@ def test_python_wrapper_keeps_reference_asserts(pkg):
         r.eval(torch.zeros(1, 9, 8, 6))
     with pytest.raises(AssertionError):
         r.eval(torch.zeros(1, 9, 6, 6))
+
+
+def test_sizes_beyond_the_limits_are_refused(lib):
+    big = lib.svbrdf_max_res() + 1
+    assert lib.svbrdf_max_res() == 7680          # 9 * 7680^2 * 4 bytes < 2^31
+    assert lib.svbrdf_fused_step_workspace_bytes(big) == 0
+    assert lib.svbrdf_render_backward_workspace_bytes(big) == 0
+    assert lib.svbrdf_render(_fake(1), big, 1, ctypes.c_float(17.0), _fake(2), _fake(3), _fake(4), _fake(5),
+                             None) == -2
+    assert lib.svbrdf_render(_fake(1), 8, lib.svbrdf_max_views() + 1, ctypes.c_float(17.0), _fake(2), _fake(3),
+                             _fake(4), _fake(5), None) == -2
+    assert _step(lib, res=big) == -2
