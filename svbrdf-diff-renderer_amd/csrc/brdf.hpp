@@ -66,9 +66,6 @@ struct Maps {
 };
 
 // 2^((A v.h + B) v.h) at v.h = 1 (co-located light and camera).
-#ifndef SVBRDF_INV_D2_FROM_RSQ
-#define SVBRDF_INV_D2_FROM_RSQ 0   // 1: 1/d^2 of a co-located view as rsq(d^2)^2 (one transcendental less)
-#endif
 
 __device__ __forceinline__ float sphg_coloc() { return fexp2(kFresA + kFresB); }
 
@@ -303,11 +300,7 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     const float sv = vx * vx + vy * vy + vz * vz;
     const float rv = frsq(sv);
     s.v[0] = vx * rv; s.v[1] = vy * rv; s.v[2] = vz * rv;
-#if SVBRDF_INV_D2_FROM_RSQ
-    s.inv_d2 = rv * rv;
-#else
     s.inv_d2 = frcp(sv);
-#endif
     s.nv_raw = m.n[0] * s.v[0] + m.n[1] * s.v[1] + m.n[2] * s.v[2];
     s.nv = tmax0(s.nv_raw);
     // GGX with 1 - (n.h)^2 = |n x v|^2 (no cancellation at the highlight)
@@ -332,7 +325,7 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
 // img = P f (n.l) / d^2.  Accumulates into `a`; a.T collects dL/dF_c, to be
 // scaled by dF/dspe = 1 - 2^(A+B) once per pixel (coloc_finish).
 __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, const float gf[3], float gnl,
-                                               Accum &a) {
+                                               float rnv, Accum &a) {
     float W;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -340,20 +333,19 @@ __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, c
         a.T[c] += gf[c] * s.DGq;
         W = c == 0 ? gf[c] * m.Fc[c] : W + gf[c] * m.Fc[c];
     }
-    // f2 = D F G / q, q = 4 nv nl + eps
-    const float Wi = W * s.iq;
-    const float gD = Wi * s.G;
-    const float gG = Wi * s.D;
-    const float gq = -Wi * s.DGq;
-    // D = a2 / (pi den^2 + eps), den = c2 a2 + 1 - c2
-    const float dD_dden = -2.0f * kPi * m.a2 * s.den * s.iqd * s.iqd;
-    const float ga2 = gD * (s.iqd + dD_dden * s.c2);
-    const float gc2 = gD * dD_dden * (m.a2 - 1.0f);
-    // n.v = n.l = n.h: q = 4 nv^2 + eps, G = G1(nv)^2, c2 = nv^2
-    const float kidv2 = m.k * s.idv * s.idv;
-    float gnv = gnl + gq * 8.0f * s.nv + gG * 2.0f * s.g1 * kidv2 + gc2 * 2.0f * s.nv;
-    const float gk = -gG * 2.0f * s.g1 * s.nv * (1.0f - s.nv) * s.idv * s.idv;
-    a.A += gk * 0.5f + ga2 * 2.0f * m.alpha;
+    // f2 = D G F / q with D, G, q functions of n.v (= n.l = n.h) and W =
+    // sum_c dL/df_c F_c; WD = W D G / q.  With rnv = 1/n.v (also used for
+    // the direct n.l term), den and iqd of GGX and idv of Smith:
+    //   d ln D / d nv = -4 pi (a2 - 1) den iqd nv    (dden/dnv = 2 (a2 - 1) nv)
+    //   d ln G / d nv = 2 k idv / nv                 (G = g1^2, g1 = nv idv)
+    //   d ln q / d nv = 8 nv iq
+    //   dL/dk        = -2 WD (1 - nv) idv            (d ln g1/dk = -(1 - nv) idv)
+    //   dL/da2       = W G iq iqd (1 - 2 pi a2 den c2 iqd)
+    const float WD = W * s.DGq;
+    const float di = s.den * s.iqd;
+    float gnv = gnl + WD * (s.nv * ((-4.0f * kPi) * (m.a2 - 1.0f) * di - 8.0f * s.iq) + 2.0f * m.k * s.idv * rnv);
+    const float ga2 = W * s.iq * s.G * s.iqd * (1.0f - (2.0f * kPi) * m.a2 * di * s.c2);
+    a.A += ga2 * 2.0f * m.alpha - WD * (1.0f - s.nv) * s.idv;
     gnv = s.nv_raw >= 0.0f ? gnv : 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) a.gn[c] += gnv * s.v[c];

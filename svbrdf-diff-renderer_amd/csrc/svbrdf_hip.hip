@@ -263,7 +263,7 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
                 gnl += gimg * P[c] * f;
                 if (LIGHT_GRAD) a.gP[c] += gimg * f * s.nl_d2;
             }
-            backward_coloc(m, s, gf, gnl * s.inv_d2, a);
+            backward_coloc(m, s, gf, gnl * s.inv_d2, frcp(fmaxf(s.nv, 1e-30f)), a);
         } else {
             Geom g;
             geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
@@ -489,7 +489,8 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
             rsum = c == 0 ? rm : rsum + rm;   // no "0 + x": it is not x for x = -0
             if (OPT_LIGHT) rP[c] += rm;
         }
-        backward_coloc(m, s, gf, K2 * rsum * frcp(fmaxf(s.nv, 1e-30f)), a);
+        const float rnv = frcp(fmaxf(s.nv, 1e-30f));
+        backward_coloc(m, s, gf, K2 * rsum * rnv, rnv, a);
     }
     coloc_finish(a);
 }
