@@ -436,7 +436,9 @@ __device__ __forceinline__ float loss_channel(float img, float y, float K2, floa
 // dL/dimg -> f, nl, P collapses: with rm = r (log-L2) or r out (L2),
 //   dL/df_c = K2 rm / f_c,  dL/dnl = K2 sum_c rm_c / nl,  dL/dP_c = K2 rm / P_c.
 // Outside the range rm = 0 (torch's clamp mask); f and nl are floored at
-// 1e-30 so that a masked zero never meets an infinite reciprocal.
+// 1e-30 so that a masked zero never meets an infinite reciprocal.  The
+// accumulators are linear in dL/df and dL/dnl, so the loop accumulates them
+// without K2 and scales them once per pixel.
 template <int LOSS, bool OPT_LIGHT, int UNROLL>
 __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, const float *__restrict__ tgt,
                                                   unsigned npix, int n_views, const float *__restrict__ cam,
@@ -485,14 +487,23 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
                 rm = inside ? r * o : 0.0f;
             }
             loss += r * r;
-            gf[c] = K2 * rm * frcp(fmaxf(f, 1e-30f));
+            // dL/df_c / K2: every accumulator is linear in these, so K2 is
+            // applied once per pixel after the loop
+            gf[c] = rm * frcp(fmaxf(f, 1e-30f));
             rsum = c == 0 ? rm : rsum + rm;   // no "0 + x": it is not x for x = -0
             if (OPT_LIGHT) rP[c] += rm;
         }
         const float rnv = frcp(fmaxf(s.nv, 1e-30f));
-        backward_coloc(m, s, gf, K2 * rsum * rnv, rnv, a);
+        backward_coloc(m, s, gf, rsum * rnv, rnv, a);
     }
     coloc_finish(a);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.S[c] *= K2;
+        a.T[c] *= K2;
+        a.gn[c] *= K2;
+    }
+    a.A *= K2;
 }
 
 template <int LOSS, bool OPT_LIGHT, bool COLOC, int NPIX, int UNROLL>
