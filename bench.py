@@ -26,9 +26,12 @@ Printed JSON line (one):
   composed          config 4: render adjoint under a VGG-shaped feature loss
   codec             the map / target codec kernels (8-bit maps and images <->
                     tensors, Lanczos-4 resize, stage transition) at the same shape
+  materialgan       the MaterialGAN caller: generator -> render -> feature loss ->
+                    latent Adam per iteration at 256^2 x 9, and the render share
 
     python bench.py [--steps K] [--warmup W] [--res 1024] [--views 9]
                     [--no-cpu] [--no-torch] [--no-batch] [--no-compose] [--no-codec]
+                    [--no-materialgan]
                     [--cpu-seconds 20] [--graph-steps 10 | --eager]
 """
 import argparse
@@ -286,6 +289,41 @@ def codec_leg(pkg, dev, res=1024, views=9, iters=20):
     return out
 
 
+def materialgan_leg(pkg, dev, res=256, views=9, iters=20):
+    """The MaterialGAN caller (src/materialgan.py:101-153) at 256^2 x 9:
+    StyleGAN2-shaped generator -> HIP render -> image MSE + 0.1 x VGG feature
+    loss -> backward to the latent -> Adam, per iteration; and the share of
+    the render forward + adjoint in it."""
+    from svbrdf_amd import materialgan as MG
+    syn = pkg.synthetic
+    cam, light = syn.flash_grid(n_views=views)
+    cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(syn.light_power())]
+    r = pkg.Microfacet(res, views, syn.PLANE_SIZE_CM, cl, dev)
+    opt = MG.MaterialGANOptim(dev, r, seed=7)
+    opt.init_from(seed=123)
+    with torch.no_grad():
+        targets = r.eval(opt.latent_to_textures(opt.latent))
+    opt.init_from(seed=5)
+    opt.load_targets(targets)
+    opt.optim((3, 2, 1), 0.02)                                      # warm-up
+    ms = time_events(lambda: opt.optim((iters, iters // 2, iters // 2), 0.02), 1) / iters
+    tex = opt.textures.clone().requires_grad_(True)
+    go = torch.randn((views, 3, res, res), device=dev)
+
+    def fb():
+        out = r.eval(tex)
+        (out * go).sum().backward()
+    for _ in range(3):
+        fb()
+    ms_render = time_events(fb, 20)
+    return {"workload": f"{res}^2 x {views} views, StyleGAN2-shaped generator (W+ 14 x 512, seeded weights), "
+                        "VGG-19 feature loss r11/r12/r32/r42 (avg pool, seeded weights), alternating latent / noise "
+                        "Adam, HIP render adjoint",
+            "iters": iters, "value": round(1000.0 / ms, 2), "unit": "it/s", "ms_per_step": round(ms, 3),
+            "render_fwd_bwd_ms": round(ms_render, 4), "render_share": round(ms_render / ms, 4),
+            "loss_image_last": opt.loss_image_list[-1]}
+
+
 def compose_leg(pkg, dev, res=256, views=9, iters=100):
     """Config 4: render -> VGG-19-shaped random-init conv stack -> Gram L2 on
     5 layers + pixel L2 -> backward (HIP adjoint) -> torch Adam, 100 steps."""
@@ -384,6 +422,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-material batched leg (config 3)")
     ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (config 4)")
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
+    ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="iterations captured per HIP graph (graph launches carry a fixed cost each)")
@@ -467,6 +506,8 @@ def main():
         out["composed"] = compose_leg(pkg, dev)
     if not args.no_codec:
         out["codec"] = codec_leg(pkg, dev, res, views)
+    if not args.no_materialgan:
+        out["materialgan"] = materialgan_leg(pkg, dev)
     if not args.no_torch:
         try:
             tb = torch_gpu_baseline(prob, res, views, dev)

@@ -8,6 +8,8 @@ Drop-in for the hot path of tflsguoyu/svbrdf-diff-renderer:
   (graph capture, exact resume)
 * :class:`BatchedStep` -- many materials of uneven size in one persistent
   launch per iteration, with an autotuned launch shape
+* :mod:`materialgan` -- the MaterialGAN caller (src/materialgan.py) on the
+  HIP render adjoint, with seeded stand-ins for the downloaded networks
 * :class:`SvbrdfIO`, :mod:`imageio` -- src/svbrdf.py:85-237 and
   src/imageio.py: maps and targets between 8-bit PNG files and tensors, the
   per-pixel codec and the Lanczos-4 resize on the GPU
@@ -16,7 +18,7 @@ The directory name is not a Python identifier; it is imported under the
 module name ``svbrdf_amd`` (see ``load_package`` in __graft_entry__.py).
 All compute runs in libsvbrdf_hip.so (gfx950); nothing falls back to the CPU.
 """
-from . import _lib, imageio, synthetic
+from . import _lib, imageio, materialgan, synthetic
 from ._lib import LOSS_L2, LOSS_LOG_L2, SvbrdfHipError
 from .batch import BatchedStep
 from .optim import FusedStep, SvbrdfOptim, loss_mode
@@ -24,4 +26,4 @@ from .renderer import Microfacet, RenderFunction
 from .svbrdf_io import SvbrdfIO, reload_textures
 
 __all__ = ["Microfacet", "RenderFunction", "SvbrdfOptim", "FusedStep", "BatchedStep", "SvbrdfIO", "reload_textures",
-           "imageio", "loss_mode", "synthetic", "LOSS_L2", "LOSS_LOG_L2", "SvbrdfHipError"]
+           "imageio", "materialgan", "loss_mode", "synthetic", "LOSS_L2", "LOSS_LOG_L2", "SvbrdfHipError"]

@@ -1,0 +1,292 @@
+"""The MaterialGAN caller of the renderer (src/materialgan.py:21-153), composed
+with the HIP render adjoint.
+
+MaterialGAN optimises a StyleGAN2 latent (W+, 14 x 512 at 256^2) and the
+per-layer noise maps so that the 9-channel SVBRDF the generator synthesises
+renders like the target photos: image MSE plus a VGG-19 feature loss
+(src/descriptor.py:7-79), alternating "L" epochs (latent, feature weights
+[1, 1, 4, 8]/14) and "N" epochs (noise, [8, 8, 2, 1]/19).
+
+The pretrained generator and VGG-19 weights are network downloads
+(src/materialgan.py:37-56, src/descriptor.py:13) and are not available
+offline, so both networks here have the published SHAPES with seeded random
+weights:
+
+* :class:`StyleGAN2Generator` -- mapping (8 x 512 fully connected, leaky
+  ReLU), W+ with 14 layers at 256^2, synthesis from a learned 4 x 4 constant
+  through modulated / demodulated 3 x 3 convolutions with noise injection,
+  x2 upsampling with the [1, 3, 3, 1] binomial filter per resolution, and the
+  "skip" output (9-channel toRGB at every resolution, upsampled and summed);
+  the architecture of Karras et al., "Analyzing and Improving the Image
+  Quality of StyleGAN" (2020), written for batch 1 (the modulation folds into
+  the convolution weights).
+* :class:`VGGLoss` -- VGG-19 features with max pooling replaced by average
+  pooling, the outputs of layers 1, 3, 13, 22 (relu1_1, relu1_2, relu3_2,
+  relu4_2) flattened, weighted and compared by MSE after ImageNet
+  normalisation (std [0.229, 0.224, 0.255] as the reference writes it).
+* :class:`MaterialGANOptim` -- init_from, load_targets, compute_feature_loss,
+  latent_to_textures (clamps of src/materialgan.py:91-100) and optim.
+
+The generator and VGG run as torch convolutions; the renderer is
+:class:`svbrdf_amd.Microfacet` (forward and adjoint kernels).
+"""
+import math
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+RES_LAYERS = {256: 14}
+_BLUR = torch.tensor([1.0, 3.0, 3.0, 1.0])
+
+
+def _lrelu(x):
+    return F.leaky_relu(x, 0.2) * math.sqrt(2.0)
+
+
+class _Dense(torch.nn.Module):
+    """Fully connected layer with the weight scale applied at run time."""
+
+    def __init__(self, din, dout, g, bias_init=0.0, lr_mul=1.0):
+        super().__init__()
+        self.weight = torch.nn.Parameter(torch.randn(dout, din, generator=g) / lr_mul)
+        self.bias = torch.nn.Parameter(torch.full((dout,), float(bias_init)))
+        self.scale = lr_mul / math.sqrt(din)
+        self.lr_mul = lr_mul
+
+    def forward(self, x):
+        return F.linear(x, self.weight * self.scale, self.bias * self.lr_mul)
+
+
+class _ModConv(torch.nn.Module):
+    """Modulated convolution for one sample: weights scaled by the style of
+    w per input channel, demodulated per output channel (optional), optional
+    x2 upsampling (zero insertion + [1,3,3,1] blur, gain 4)."""
+
+    def __init__(self, cin, cout, k, g, demodulate=True, upsample=False):
+        super().__init__()
+        self.weight = torch.nn.Parameter(torch.randn(cout, cin, k, k, generator=g))
+        self.affine = _Dense(512, cin, g, bias_init=1.0)
+        self.scale = 1.0 / math.sqrt(cin * k * k)
+        self.demodulate, self.upsample, self.k = demodulate, upsample, k
+
+    def forward(self, x, w):
+        style = self.affine(w)                                       # [1, cin]
+        wt = self.weight * self.scale * style.view(1, -1, 1, 1)
+        if self.demodulate:
+            wt = wt * torch.rsqrt((wt * wt).sum(dim=(1, 2, 3), keepdim=True) + 1e-8)
+        if self.upsample:
+            x = _upsample2(x)
+        return F.conv2d(x, wt, padding=self.k // 2)
+
+
+def _upsample2(x):
+    """x2 upsampling: zeros inserted between pixels, then the separable
+    [1, 3, 3, 1] binomial filter with gain 4 (unit DC response)."""
+    n, c, h, w = x.shape
+    up = x.new_zeros(n, c, 2 * h, 2 * w)
+    up[:, :, ::2, ::2] = x
+    k = (_BLUR / _BLUR.sum() * 2.0).to(x)                            # per axis, gain 2
+    up = F.pad(up, (2, 1, 2, 1), mode="constant")
+    up = F.conv2d(up, k.view(1, 1, 1, 4).repeat(c, 1, 1, 1), groups=c)
+    return F.conv2d(up, k.view(1, 1, 4, 1).repeat(c, 1, 1, 1), groups=c)
+
+
+class StyleGAN2Generator(torch.nn.Module):
+    """StyleGAN2 ("skip") generator shape at 256^2 with 9 output channels;
+    ``mapping``, ``truncation`` and ``synthesis`` as the reference's net
+    attributes (src/materialgan.py:61-62,92)."""
+
+    def __init__(self, resolution=256, channels=9, seed=7, device="cuda"):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        self.resolution, self.channels = resolution, channels
+        self.num_layers = RES_LAYERS[resolution]
+        nf = lambda r: min(32768 // r, 512)
+        self.map_layers = torch.nn.ModuleList([_Dense(512, 512, g, lr_mul=0.01) for _ in range(8)])
+        self.const = torch.nn.Parameter(torch.randn(1, nf(4), 4, 4, generator=g))
+        self.convs = torch.nn.ModuleList([_ModConv(nf(4), nf(4), 3, g)])
+        self.to_rgb = torch.nn.ModuleList([_ModConv(nf(4), channels, 1, g, demodulate=False)])
+        self.noise_strength = torch.nn.ParameterList([torch.nn.Parameter(torch.tensor(0.1))])
+        self.conv_res = [4]
+        r = 8
+        while r <= resolution:
+            self.convs.append(_ModConv(nf(r // 2), nf(r), 3, g, upsample=True))
+            self.convs.append(_ModConv(nf(r), nf(r), 3, g))
+            self.to_rgb.append(_ModConv(nf(r), channels, 1, g, demodulate=False))
+            self.noise_strength.append(torch.nn.Parameter(torch.tensor(0.1)))
+            self.noise_strength.append(torch.nn.Parameter(torch.tensor(0.1)))
+            self.conv_res += [r, r]
+            r *= 2
+        self.bias = torch.nn.ParameterList([torch.nn.Parameter(torch.zeros(c.weight.shape[0])) for c in self.convs])
+        self.register_buffer("w_avg", torch.zeros(512))
+        self.to(device)
+        for p in self.parameters():
+            p.requires_grad_(False)
+
+    def mapping(self, z):
+        x = z * torch.rsqrt((z * z).mean(dim=1, keepdim=True) + 1e-8)
+        for layer in self.map_layers:
+            x = _lrelu(layer(x))
+        return x
+
+    def truncation(self, w, psi=1.0):
+        """W -> W+ ([1, num_layers, 512]); psi = 1 keeps w (as the reference's
+        generator is configured, truncation_psi = 1.0)."""
+        wp = w.view(-1, 1, 512).repeat(1, self.num_layers, 1)
+        return self.w_avg.view(1, 1, 512) + psi * (wp - self.w_avg.view(1, 1, 512))
+
+    def make_noises(self, gen=None):
+        """One [1, 1, r, r] noise map per convolution layer."""
+        return [torch.randn((1, 1, r, r), generator=gen).to(self.const.device) for r in self.conv_res]
+
+    def synthesis(self, wp, noises):
+        """W+ [1, num_layers, 512] and the per-layer noise maps -> [1, C, R, R]."""
+        x = self.const
+        rgb = None
+        wi = 0
+        for i, conv in enumerate(self.convs):
+            x = conv(x, wp[:, min(wi, self.num_layers - 1)])
+            x = x + self.noise_strength[i] * noises[i]
+            x = _lrelu(x + self.bias[i].view(1, -1, 1, 1))
+            wi += 1
+            if i == 0 or i % 2 == 0:
+                # toRGB after each resolution's last convolution, skip-summed
+                t = self.to_rgb[(i + 1) // 2](x, wp[:, min(wi, self.num_layers - 1)])
+                rgb = t if rgb is None else _upsample2(rgb) + t
+        return rgb
+
+
+class VGGLoss(torch.nn.Module):
+    """src/descriptor.py:7-79 with seeded random VGG-19 weights."""
+
+    LAYERS = [64, 64, "P", 128, 128, "P", 256, 256, 256, 256, "P", 512, 512]   # conv1_1 .. conv4_2
+    HOOKS = (1, 3, 13, 22)                                                   # relu1_1, relu1_2, relu3_2, relu4_2
+
+    def __init__(self, device, weights=np.array([1, 1, 1, 1]) / 4, seed=4):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        mods, cin = [], 3
+        for item in self.LAYERS:
+            if item == "P":
+                mods.append(torch.nn.AvgPool2d(kernel_size=2))       # max pooling replaced, descriptor.py:18-20
+                continue
+            conv = torch.nn.Conv2d(cin, item, 3, padding=1)
+            with torch.no_grad():
+                conv.weight.copy_(torch.randn(conv.weight.shape, generator=g) * math.sqrt(2.0 / (cin * 9)))
+                conv.bias.zero_()
+            mods += [conv, torch.nn.ReLU()]
+            cin = item
+        self.net = torch.nn.Sequential(*mods).to(device)
+        for p in self.net.parameters():
+            p.requires_grad_(False)
+        self.weights = [float(x) for x in weights]
+        self.mean = torch.tensor([0.485, 0.456, 0.406], device=device).view(1, 3, 1, 1)
+        self.std = torch.tensor([0.229, 0.224, 0.255], device=device).view(1, 3, 1, 1)
+
+    def normalize(self, im):
+        return (im - self.mean) / self.std
+
+    def compute_feature_vector(self, x):
+        feats = []
+        for i, layer in enumerate(self.net):
+            x = layer(x)
+            if i in self.HOOKS:
+                feats.append(x.flatten() * self.weights[len(feats)])
+        return torch.cat(feats)
+
+    def load(self, im):
+        with torch.no_grad():
+            self.im_feature = self.compute_feature_vector(self.normalize(im))
+
+    def forward(self, x):
+        return F.mse_loss(self.compute_feature_vector(self.normalize(x)), self.im_feature)
+
+
+class MaterialGANOptim:
+    """src/materialgan.py:21-153 (checkpoints replaced by seeded weights)."""
+
+    def __init__(self, device, renderer_obj, ckp=None, seed=7):
+        self.device = torch.device(device)
+        self.renderer_obj = renderer_obj
+        self.net_obj = StyleGAN2Generator(renderer_obj.res, 9, seed=seed, device=self.device)
+        self.loss_large_feature = VGGLoss(self.device, np.array([1, 1, 4, 8]) / 14)
+        self.loss_small_feature = VGGLoss(self.device, np.array([8, 8, 2, 1]) / 19)
+        self.noises = None
+        self.latent = None
+
+    def init_from(self, ckp=(), seed=0):
+        """Latent W+ from a random z through mapping + truncation (or a saved
+        tensor), and the noise maps (src/materialgan.py:58-75)."""
+        gen = torch.Generator().manual_seed(seed)
+        if len(ckp) == 0:
+            z = torch.randn((1, 512), generator=gen).to(self.device)
+            with torch.no_grad():
+                latent = self.net_obj.truncation(self.net_obj.mapping(z))
+        else:
+            latent = torch.load(ckp[0], map_location=self.device, weights_only=True)
+        self.latent = latent.detach().clone().requires_grad_(True)
+        if len(ckp) == 2:
+            self.noises = [n.to(self.device).requires_grad_(True)
+                           for n in torch.load(ckp[1], map_location=self.device, weights_only=True)]
+        else:
+            self.noises = [n.requires_grad_(True) for n in self.net_obj.make_noises(gen)]
+
+    def load_targets(self, targets):
+        self.targets = targets
+        self.loss_large_feature.load(targets)
+        self.loss_small_feature.load(targets)
+
+    def compute_image_loss(self, predicts):
+        return F.mse_loss(predicts, self.targets)
+
+    def compute_feature_loss(self, predicts, flag):
+        if flag == "L":
+            return self.loss_large_feature(predicts)
+        if flag == "N":
+            return self.loss_small_feature(predicts)
+        raise ValueError("a feature-loss flag is needed, 'L' or 'N'")
+
+    def latent_to_textures(self, latent):
+        t = self.net_obj.synthesis(latent, self.noises)
+        textures = t.clone()
+        textures[:, 0:5] = t[:, 0:5].clamp(-1, 1)
+        textures[:, 5] = t[:, 5].clamp(-0.5, 0.5)
+        textures[:, 6:9] = t[:, 6:9].clamp(-1, 1)
+        return textures
+
+    def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False):
+        """epochs = (total, l_epochs, n_epochs): l_epochs optimising the latent
+        (feature loss "L") then n_epochs the noise ("N"), repeated; a fresh
+        Adam each epoch as the reference builds it (src/materialgan.py:
+        114-126).  Returns the per-epoch (image, feature) losses."""
+        total, l_epochs, n_epochs = epochs
+        cycle = l_epochs + n_epochs
+        light = None
+        if optim_light:
+            light = (svbrdf_obj.cl[2] if svbrdf_obj is not None else self.renderer_obj.light_pow)
+            light = light.detach().to(self.device, torch.float32).reshape(3).clone().requires_grad_(True)
+            if svbrdf_obj is not None:
+                svbrdf_obj.cl[2] = light
+        image_losses, feature_losses = [], []
+        for epoch in range(total):
+            which = "L" if (epoch % cycle) < l_epochs else "N"
+            params = [self.latent] if which == "L" else list(self.noises)
+            if light is not None:
+                params = params + [light]
+                self.renderer_obj.update_light(light)
+            optimizer = torch.optim.Adam(params, lr=lr, betas=(0.9, 0.999))
+            textures = self.latent_to_textures(self.latent)
+            rendereds = self.renderer_obj.eval(textures)
+            loss_image = self.compute_image_loss(rendereds)
+            loss_feature = self.compute_feature_loss(rendereds, which) * 0.1
+            loss = loss_image + loss_feature
+            optimizer.zero_grad()
+            loss.backward()
+            optimizer.step()
+            image_losses.append(loss_image.detach())
+            feature_losses.append(loss_feature.detach())
+        self.textures = textures.detach()
+        self.loss_image_list = [float(x) for x in torch.stack(image_losses).cpu()]
+        self.loss_feature_list = [float(x) for x in torch.stack(feature_losses).cpu()]
+        return self.loss_image_list, self.loss_feature_list
