@@ -57,8 +57,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per fused-step launch read from the profiler (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, in KiB per dispatch, FETCH_SIZE scaled by 2.0 as a
 # plain device copy of known size calibrates it, WRITE_SIZE by 1.0), for the
-# shapes that were profiled: 110,695 KiB x 2 + 110,852 KiB at 1024^2 x 9.
-MEASURED_TRAFFIC = {(1024, 9): (110695 * 2 + 110852) * 1024}
+# shapes that were profiled: 110,690 KiB x 2 + 110,852 KiB at 1024^2 x 9.
+MEASURED_TRAFFIC = {(1024, 9): (110690 * 2 + 110852) * 1024}
 
 
 def algorithmic_bytes_step(res, views):
