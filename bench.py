@@ -3,7 +3,7 @@
 One step = one fused iteration of the reference loop body (src/svbrdf.py:58-71):
 clamp -> render -> log-space L2 loss -> backward -> Adam, on a 1024x1024
 material seen from 9 co-located flash views, all fp32, as one kernel launch
-per iteration, replayed from HIP graphs of 10 iterations.  Inputs are
+per iteration, replayed from HIP graphs of 100 iterations.  Inputs are
 synthetic (seeded maps, targets rendered from the ground truth) and resident
 in HBM before timing starts.
 
@@ -16,9 +16,13 @@ Printed JSON line (one):
                     around single launches), against 8 TB/s; step_frac is the
                     same over the whole-job time per iteration; traffic is the
                     profiler-measured HBM bytes per launch
+  per_replay_us_per_step  median and IQR over 50 individually timed graph replays
+  with_sync         it/s with the loss read on the host every iteration, as the
+                    reference loop does (src/svbrdf.py:64)
   torch_gpu_baseline  the reference's eager op sequence + torch.optim.Adam on
                     the same GPU (it/s), and the speedup over it
-  cpu_baseline      the same eager restatement on the host CPU cores (oracle/)
+  cpu_baseline      the same eager restatement on the host CPU cores (oracle/);
+                    cpu_baseline_256 the same at 256^2 (config 0)
   forward           forward-render throughput at the same shape (config 1)
 
   batched           config 3: 64 materials of uneven size in one persistent
@@ -100,6 +104,38 @@ def kernel_duration_ms(launch, n):
         b.record()
     torch.cuda.synchronize()
     return sum(a.elapsed_time(b) for a, b in ev) / n
+
+
+def replay_spread(run, per, n):
+    """Median and interquartile range (us per iteration) over n single
+    timed units of `per` iterations (one graph replay, or one eager launch)."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    torch.cuda.synchronize()
+    for a, b in ev:
+        a.record()
+        run(per)
+        b.record()
+    torch.cuda.synchronize()
+    us = np.array([a.elapsed_time(b) * 1000.0 / per for a, b in ev])
+    q1, med, q3 = np.percentile(us, [25, 50, 75])
+    return {"median": round(float(med), 2), "iqr": round(float(q3 - q1), 2), "n": n, "iterations_each": per}
+
+
+def with_sync_rate(fs, n):
+    """Iterations/sec when every iteration's loss is read on the host, as the
+    reference loop does (loss.item(), src/svbrdf.py:64): one launch, the
+    flush and a device-to-host copy of the state per iteration."""
+    for _ in range(3):
+        fs.step()
+        fs.loss()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fs.step()
+        fs.loss()
+    dt = (time.perf_counter() - t0) / n
+    return {"value": round(1.0 / dt, 1), "unit": "it/s", "ms_per_step": round(dt * 1000.0, 4),
+            "what": "eager launch + loss flush + host read of the loss every iteration (src/svbrdf.py:64)"}
 
 
 def time_events(fn, n, stream=None):
@@ -424,8 +460,9 @@ def main():
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
-    ap.add_argument("--graph-steps", type=int, default=10,
-                    help="iterations captured per HIP graph (graph launches carry a fixed cost each)")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="iterations captured per HIP graph (each graph launch costs ~35 us on the GPU, so "
+                         "a long optimisation loop is captured in large pieces)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the bench, run 5 plain device copies of MIB MiB (a known-size kernel that "
                          "calibrates the profiler's FETCH_SIZE / WRITE_SIZE counters)")
@@ -467,6 +504,8 @@ def main():
     loss_after = fs.loss()
     value = 1000.0 / ms
     kernel_ms = kernel_duration_ms(fs.step, 50)
+    spread = replay_spread(run, args.graph_steps if not args.eager else 1, 50)
+    with_sync = with_sync_rate(fs, 50)
 
     nbytes = algorithmic_bytes_step(res, views)
     achieved = nbytes / (kernel_ms * 1e-3) / 1e9
@@ -486,6 +525,8 @@ def main():
                      "kernel_ms": round(kernel_ms, 5),
                      "step_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "loss_after": loss_after,
+        "per_replay_us_per_step": spread,
+        "with_sync": with_sync,
     }
 
     # forward render at the same shape (config 1)
@@ -518,6 +559,8 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(res, views, args.cpu_seconds)
         out["cpu_baseline"] = cb
+        # config 0 (256^2 x 9, the reference's CPU-runnable case) as well
+        out["cpu_baseline_256"] = cpu_baseline(256, views, min(args.cpu_seconds, 5.0))
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
