@@ -30,11 +30,15 @@ weights:
 The generator and VGG run as torch convolutions; the renderer is
 :class:`svbrdf_amd.Microfacet` (forward and adjoint kernels).
 """
+import datetime
 import math
+import pathlib
 
 import numpy as np
 import torch
 import torch.nn.functional as F
+
+from .optim import SvbrdfOptim
 
 RES_LAYERS = {256: 14}
 _BLUR = torch.tensor([1.0, 3.0, 3.0, 1.0])
@@ -255,13 +259,25 @@ class MaterialGANOptim:
         textures[:, 6:9] = t[:, 6:9].clamp(-1, 1)
         return textures
 
-    def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False):
+    save_loss = staticmethod(SvbrdfOptim.save_loss)
+
+    def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False, save_every=200):
         """epochs = (total, l_epochs, n_epochs): l_epochs optimising the latent
         (feature loss "L") then n_epochs the noise ("N"), repeated; a fresh
         Adam each epoch as the reference builds it (src/materialgan.py:
-        114-126).  Returns the per-epoch (image, feature) losses."""
+        114-126).  With an ``svbrdf_obj`` that has ``optimize_dir``, the loss
+        plot, latent, noise, maps and re-renders are saved after epoch 1,
+        every 200th and the last under optimize_dir/tmp/<timestamp>/<epoch>/
+        (:103-105, 157-169).  Afterwards ``textures`` holds the maps of the
+        final latent and noise, and ``loss`` / ``loss_image`` the last epoch's
+        losses (:171-173).  Returns the per-epoch (image, feature) losses."""
         total, l_epochs, n_epochs = epochs
         cycle = l_epochs + n_epochs
+        tmp_dir = None
+        if svbrdf_obj is not None and hasattr(svbrdf_obj, "optimize_dir"):
+            stamp = str(datetime.datetime.now()).replace(" ", "-").replace(":", "-").replace(".", "-")
+            tmp_dir = pathlib.Path(svbrdf_obj.optimize_dir) / "tmp" / stamp
+            tmp_dir.mkdir(parents=True, exist_ok=True)
         light = None
         if optim_light:
             light = (svbrdf_obj.cl[2] if svbrdf_obj is not None else self.renderer_obj.light_pow)
@@ -286,7 +302,26 @@ class MaterialGANOptim:
             optimizer.step()
             image_losses.append(loss_image.detach())
             feature_losses.append(loss_feature.detach())
-        self.textures = textures.detach()
-        self.loss_image_list = [float(x) for x in torch.stack(image_losses).cpu()]
-        self.loss_feature_list = [float(x) for x in torch.stack(feature_losses).cpu()]
+            if tmp_dir is not None and ((epoch + 1) % save_every == 0 or epoch == 0 or epoch == total - 1):
+                this_dir = tmp_dir / f"{epoch + 1}"
+                this_dir.mkdir(parents=True, exist_ok=True)
+                self.save_loss([torch.stack(image_losses).cpu().numpy(), torch.stack(feature_losses).cpu().numpy()],
+                               ["image loss", "feature loss"], tmp_dir / "loss.jpg", total)
+                torch.save(self.latent.detach(), this_dir / "optim_latent.pt")
+                torch.save([n.detach() for n in self.noises], this_dir / "optim_noise.pt")
+                with torch.no_grad():
+                    saved = self.latent_to_textures(self.latent)
+                    if hasattr(svbrdf_obj, "save_textures_th"):
+                        svbrdf_obj.save_textures_th(saved, this_dir)
+                    if hasattr(svbrdf_obj, "save_images_th"):
+                        svbrdf_obj.save_images_th(self.renderer_obj.eval(saved), this_dir)
+        if total > 0:
+            # the reference leaves the maps of the final latent and noise here
+            # (its last epoch is always a save epoch, which re-synthesises them)
+            with torch.no_grad():
+                self.textures = self.latent_to_textures(self.latent).detach()
+            self.loss = float(loss.detach())
+            self.loss_image = float(loss_image)
+        self.loss_image_list = [float(x) for x in torch.stack(image_losses).cpu()] if image_losses else []
+        self.loss_feature_list = [float(x) for x in torch.stack(feature_losses).cpu()] if feature_losses else []
         return self.loss_image_list, self.loss_feature_list

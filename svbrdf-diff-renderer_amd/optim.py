@@ -42,6 +42,24 @@ def loss_mode(loss):
         raise ValueError(f"unknown loss {loss!r}; expected one of {sorted(LOSSES)}") from None
 
 
+def epoch_segments(epochs, save_every=100, sync_every=0, saving=True):
+    """Splits an optimisation of `epochs` iterations at the epochs after which
+    the host acts: [(first, last, save)], 0-based and inclusive.  `save` marks
+    the reference's save points -- epoch 1, every `save_every`-th and the last
+    (src/svbrdf.py:74) -- when `saving`; a segment also ends every
+    `sync_every` epochs when that is set.  The iterations of a segment need no
+    host work between them."""
+    if save_every < 1:
+        raise ValueError("save_every must be >= 1")
+    out, first = [], 0
+    for e in range(epochs):
+        save = saving and ((e + 1) % save_every == 0 or e == 0 or e == epochs - 1)
+        if save or (sync_every and (e + 1) % sync_every == 0) or e == epochs - 1:
+            out.append((first, e, save))
+            first = e + 1
+    return out
+
+
 class FusedStep:
     """One optimisation problem: textures (updated in place) against targets.
 
@@ -93,6 +111,7 @@ class FusedStep:
         self.steps_launched = 0
         self.graph = None
         self.graph_steps = 0
+        self._graphs = {}
 
     def step(self):
         """Launch one fused iteration on the current stream (asynchronous)."""
@@ -109,6 +128,32 @@ class FusedStep:
     def capture(self, steps=1):
         """Record `steps` consecutive iterations into one HIP graph
         (torch.cuda.CUDAGraph); every replay() then runs that many."""
+        self.graph = self._capture_graph(steps)
+        self.graph_steps = steps
+        return self.graph
+
+    def run(self, n, graph_steps=100):
+        """n iterations as replays of cached graphs of `graph_steps`
+        iterations (and one of the remainder); graph_steps=0 launches them
+        one by one.  Each graph launch costs the GPU some tens of
+        microseconds, so long loops go in large pieces."""
+        if n <= 0:
+            return
+        if not graph_steps:
+            for _ in range(n):
+                self.step()
+            return
+        per = max(1, min(int(graph_steps), int(n)))
+        for size in {per, n % per} - {0}:
+            if size not in self._graphs:
+                self._graphs[size] = self._capture_graph(size)
+        for _ in range(n // per):
+            self._graphs[per].replay()
+        if n % per:
+            self._graphs[n % per].replay()
+        self.steps_launched += n
+
+    def _capture_graph(self, steps):
         if steps < 1:
             raise ValueError("steps must be >= 1")
         g = torch.cuda.CUDAGraph()
@@ -127,8 +172,6 @@ class FusedStep:
                           _lib.ptr(self.workspace), self.workspace.numel() * 4, ctypes.c_void_p(s.cuda_stream))
             g.capture_end()
         torch.cuda.current_stream().wait_stream(s)
-        self.graph = g
-        self.graph_steps = steps
         return g
 
     def replay(self):
@@ -225,15 +268,18 @@ class SvbrdfOptim:
         return torch.nn.functional.mse_loss(torch.log(predicts), torch.log(self.targets.clamp(floor, 1)))
 
     def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False, loss=None, save_every=100,
-              sync_every=0):
+              sync_every=0, graph_steps=100):
         """src/svbrdf.py:44-83.
 
-        The loop body runs as one fused kernel per epoch.  Side outputs are as
-        in the reference when ``svbrdf_obj`` provides them (maps and
-        re-renders saved at epoch 1, every ``save_every`` epochs and the last
-        epoch under optimize_dir/tmp/<timestamp>/<epoch>/).  Losses are
-        recorded on the device and read back at the end (or every
-        ``sync_every`` epochs), instead of with one ``.item()`` per epoch.
+        The loop body runs as one fused kernel per epoch; the epochs between
+        two host events (a save, a sync) run as replays of HIP graphs of
+        ``graph_steps`` epochs.  Side outputs are as in the reference when
+        ``svbrdf_obj`` provides them (maps and re-renders saved at epoch 1,
+        every ``save_every`` epochs and the last epoch under
+        optimize_dir/tmp/<timestamp>/<epoch>/).  Losses are recorded on the
+        device and read back at the end (the host also synchronises every
+        ``sync_every`` epochs when that is set), instead of with one
+        ``.item()`` per epoch.
         """
         if loss is not None:
             self.loss = loss
@@ -256,13 +302,14 @@ class SvbrdfOptim:
             stamp = str(datetime.datetime.now()).replace(" ", "-").replace(":", "-").replace(".", "-")
             tmp_dir = pathlib.Path(svbrdf_obj.optimize_dir) / "tmp" / stamp
             tmp_dir.mkdir(parents=True, exist_ok=True)
-        for epoch in range(epochs):
-            self.fused.step()
-            if sync_every and (epoch + 1) % sync_every == 0:
+        for first, last, save in epoch_segments(epochs, save_every, sync_every, tmp_dir is not None):
+            self.fused.run(last - first + 1, graph_steps)
+            if sync_every:
                 torch.cuda.synchronize(self.device)
-            if tmp_dir is not None and ((epoch + 1) % save_every == 0 or epoch == 0 or epoch == epochs - 1):
-                this_dir = tmp_dir / f"{epoch + 1}"
+            if save:
+                this_dir = tmp_dir / f"{last + 1}"
                 this_dir.mkdir(parents=True, exist_ok=True)
+                self.save_loss([self.fused.losses()], ["image loss"], tmp_dir / "loss.jpg", epochs)
                 with torch.no_grad():
                     tc = self.textures.detach().clamp(-1, 1)
                     if hasattr(svbrdf_obj, "save_textures_th"):
@@ -271,3 +318,25 @@ class SvbrdfOptim:
                         svbrdf_obj.save_images_th(r.eval(tc), this_dir)
         self.loss_image_list = [float(x) for x in self.fused.losses()]
         return self.loss_image_list
+
+    def gradient(self, parameters):
+        """src/optimization.py:17-23: the tensor (or each tensor of a list)
+        with requires_grad set."""
+        if isinstance(parameters, list):
+            return [p.detach().requires_grad_(True) for p in parameters]
+        return parameters.detach().requires_grad_(True)
+
+    @staticmethod
+    def save_loss(losses, labels, save_dir, N):
+        """src/optimization.py:35-43: log(1 + loss) curves over N epochs."""
+        import matplotlib
+        matplotlib.use("Agg")
+        import matplotlib.pyplot as plt
+        plt.figure(figsize=(8, 4))
+        for loss, label in zip(losses, labels):
+            plt.plot(np.log1p(np.asarray(loss, dtype=np.float64)), label=label)
+        plt.xlim(0, N)
+        plt.legend()
+        plt.title("log(1+loss)")
+        plt.savefig(save_dir)
+        plt.close()

@@ -90,3 +90,22 @@ def test_bench_eager_baseline_renders_the_reference_images():
     cl = [torch.tensor(cam), torch.tensor(cam), torch.tensor(P)]
     ref32 = MT.MicrofacetEager(res, V, 17.0, cl, "cpu").eval(torch.tensor(tex)).numpy()
     np.testing.assert_allclose(out, ref32, rtol=2e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("epochs,save_every,sync_every", [(1, 100, 0), (2, 100, 0), (100, 100, 0),
+                                                          (250, 100, 0), (1000, 100, 64), (37, 5, 7)])
+def test_epoch_segments_cover_the_loop_and_end_at_save_points(pkg, epochs, save_every, sync_every):
+    segs = pkg.optim.epoch_segments(epochs, save_every, sync_every)
+    # contiguous, in order, covering every epoch once
+    assert segs[0][0] == 0 and segs[-1][1] == epochs - 1
+    assert all(a[1] + 1 == b[0] for a, b in zip(segs, segs[1:]))
+    saves = [last + 1 for _, last, save in segs if save]
+    # the reference saves after epoch 1, every 100th and the last (src/svbrdf.py:74)
+    want = sorted({1, epochs} | set(range(save_every, epochs + 1, save_every)))
+    assert saves == want
+    if sync_every:
+        ends = {last + 1 for _, last, _ in segs}
+        assert set(range(sync_every, epochs + 1, sync_every)) <= ends
+        assert all(last - first + 1 <= sync_every for first, last, _ in segs)
+    # without saving or syncing the whole run is one segment
+    assert pkg.optim.epoch_segments(epochs, save_every, 0, saving=False) == [(0, epochs - 1, False)]
