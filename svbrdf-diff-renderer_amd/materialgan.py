@@ -321,7 +321,7 @@ class MaterialGANOptim:
             with torch.no_grad():
                 self.textures = self.latent_to_textures(self.latent).detach()
             self.loss = float(loss.detach())
-            self.loss_image = float(loss_image)
+            self.loss_image = float(loss_image.detach())
         self.loss_image_list = [float(x) for x in torch.stack(image_losses).cpu()] if image_losses else []
         self.loss_feature_list = [float(x) for x in torch.stack(feature_losses).cpu()] if feature_losses else []
         return self.loss_image_list, self.loss_feature_list

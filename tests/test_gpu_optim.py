@@ -161,6 +161,15 @@ def test_scripts_render_then_optim_perpixel(pkg, tmp_path):
     assert len(runs) == 2
     assert sorted(p.name for p in runs[-1].iterdir() if p.is_dir()) == ["1", "100", "120"]
     assert (runs[-1] / "120" / "tex.png").is_file() and (runs[-1] / "loss.jpg").is_file()
+    # coarse-to-fine (run.py:55-56): the next stage reads this stage's saved
+    # maps at twice the resolution (Lanczos-4 resize of maps and targets)
+    d = json.loads(cfg.read_text())
+    d.update(reference_dir="opt", optimize_dir="opt2", rerender_dir="rr2")
+    (tmp_path / "cfg2.json").write_text(json.dumps(d))
+    opt2 = pkg.scripts.optim_perpixel(tmp_path / "cfg2.json", 2 * res, 0.01, 5, "textures")
+    assert opt2.textures.shape == (1, 9, 2 * res, 2 * res)
+    assert opt2.loss_image_list[0] < 0.5 * li[0]
+    assert (tmp_path / "opt2" / "tex.png").is_file()
     with pytest.raises(ValueError):
         pkg.scripts.optim_perpixel(cfg, res, 0.01, 1, "bogus")
 

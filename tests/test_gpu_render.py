@@ -71,7 +71,8 @@ def test_adjoint_matches_reference_autograd_golden(pkg, golden_dir, name):
 
 
 @pytest.mark.parametrize("res,V,coloc", [(1, 1, True), (2, 3, False), (13, 9, True), (64, 9, True),
-                                         (64, 16, False), (100, 4, True), (256, 9, True)])
+                                         (64, 16, False), (100, 4, True), (256, 9, True),
+                                         (5, 1024, True), (3, 1024, False)])
 def test_forward_and_adjoint_synthetic(pkg, res, V, coloc):
     args = scene(pkg, res, V, seed=res + V, coloc=coloc)
     gout = np.random.default_rng(res).standard_normal((V, 3, res, res)).astype(np.float32)

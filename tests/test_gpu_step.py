@@ -36,7 +36,8 @@ def eager_loss_grad(tex, target, res, size, cam, light, P, mode, dtype):
 
 
 @pytest.mark.parametrize("loss,mode", [("l2", M.LOSS_L2), ("log_l2", M.LOSS_LOG_L2)])
-@pytest.mark.parametrize("res,V,coloc", [(64, 9, True), (33, 5, False), (256, 9, True)])
+@pytest.mark.parametrize("res,V,coloc", [(64, 9, True), (33, 5, False), (256, 9, True), (1, 1, True),
+                                         (4, 1024, True), (3, 1024, False)])
 def test_first_step_gradient_and_loss(pkg, loss, mode, res, V, coloc):
     gt, _, size, cam, light, P = scene(pkg, res, V, seed=11, coloc=coloc)
     init = pkg.synthetic.init_textures(gt, seed=1)
