@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 2
+#define SVBRDF_ABI_VERSION 3
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -152,14 +152,27 @@ int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream
  * finalize follows the step kernel (the next step's light power needs this
  * step's full dL/dP) and records the loss at once.
  *
+ * Layout: every plane of textures ([9] planes), exp_avg, exp_avg_sq ([9]) and
+ * targets_prepared ([3 V]) holds R*R floats; consecutive planes of textures
+ * start texture_stride floats apart, those of the other three plane_stride
+ * floats apart (0 means R*R: dense [.., R, R] tensors as torch holds them).
+ * Buffers the caller allocates for the fused step alone should use
+ * svbrdf_plane_stride(res): at R = 1024 the dense plane spacing is exactly
+ * 4 MiB, and the 54 planes a wave touches then fall on the same HBM
+ * channels (about 7 % slower).
+ *
  * Workspace: svbrdf_fused_step_workspace_bytes(res) bytes, 256-byte aligned,
  * zero-filled before the first call and then kept with `state` (it holds
  * arrival counters, which every launch leaves at zero, and the partial sums
  * of the last step).  One state + workspace pair serves one problem (one
  * res); calls that share a pair must be stream-ordered. */
 size_t svbrdf_fused_step_workspace_bytes(int res);
+/* Plane stride (in floats) for buffers laid out for the fused step: R*R
+ * rounded up to 64 floats (256-byte aligned planes) plus 448 floats. */
+size_t svbrdf_plane_stride(int res);
 int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq,
-                      const float *targets_prepared, int res, int n_views, float size,
+                      const float *targets_prepared, int res, size_t texture_stride,
+                      size_t plane_stride, int n_views, float size,
                       const float *camera_pos, const float *light_pos, float *light_pow,
                       int loss_mode, int optim_light, double lr, double beta1,
                       double beta2, double eps, void *state, float *loss_history,
@@ -256,7 +269,9 @@ typedef struct svbrdf_material {
     float size;                     /* plane width, cm                              */
     unsigned int tile_begin;        /* set by svbrdf_batch_plan                     */
     unsigned int n_tiles;           /* set by svbrdf_batch_plan                     */
-    unsigned int reserved;
+    unsigned int plane_stride;      /* floats between the planes of exp_avg,
+                                       exp_avg_sq and targets_prepared; 0 = R*R
+                                       (textures are always dense)               */
 } svbrdf_material;
 
 /* Number of tiles of `tile_pixels` pixels (a multiple of 256) the batch splits

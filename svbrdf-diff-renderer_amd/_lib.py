@@ -44,7 +44,9 @@ SIGNATURES = {
     "svbrdf_prepare_targets": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "svbrdf_step_state_init": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp]),
     "svbrdf_fused_step_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),
-    "svbrdf_fused_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp,
+    "svbrdf_plane_stride": (ctypes.c_size_t, [ctypes.c_int]),
+    "svbrdf_fused_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.c_int, ctypes.c_float, _vp, _vp,
                                          _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, _vp, _vp, ctypes.c_int, _vp,
                                          ctypes.c_size_t, _vp]),

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .optim import _STATE_DTYPE, loss_mode
+from .optim import _STATE_DTYPE, _planes, loss_mode
 
 DEFAULT_CONFIG = {"tile_pixels": 4096, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": 0}
 
@@ -34,7 +34,7 @@ class Material(ctypes.Structure):
                 ("targets_prepared", ctypes.c_void_p), ("camera_pos", ctypes.c_void_p),
                 ("light_pos", ctypes.c_void_p), ("light_pow", ctypes.c_void_p), ("res", ctypes.c_int),
                 ("n_views", ctypes.c_int), ("size", ctypes.c_float), ("tile_begin", ctypes.c_uint),
-                ("n_tiles", ctypes.c_uint), ("reserved", ctypes.c_uint)]
+                ("n_tiles", ctypes.c_uint), ("plane_stride", ctypes.c_uint)]
 
 
 def _cache_path():
@@ -63,18 +63,26 @@ class BatchedStep:
         self.mode = loss_mode(loss)
         s = _lib.stream_handle(self.device)
         self.renderers, self.textures, self.exp_avg, self.exp_avg_sq, self.targets = [], [], [], [], []
+        self.plane_strides = []
         for r, tex, tgt in problems:
             if tex.device != self.device or tex.dtype != torch.float32 or not tex.is_contiguous():
                 raise ValueError("textures must be contiguous float32 tensors on one CUDA device")
             if tex.numel() != 9 * r.res * r.res or tuple(tgt.shape) != (r.n_of_imgs, 3, r.res, r.res):
                 raise ValueError(f"material of res {r.res} / {r.n_of_imgs} views: shape mismatch")
             tg = tgt.detach().to(self.device, torch.float32).contiguous()
-            prepared = torch.empty_like(tg)
-            _lib.call("svbrdf_prepare_targets", _lib.ptr(tg), tg.numel(), self.mode, _lib.ptr(prepared), s)
+            dense = torch.empty_like(tg)
+            _lib.call("svbrdf_prepare_targets", _lib.ptr(tg), tg.numel(), self.mode, _lib.ptr(dense), s)
+            # moments and prepared targets at the library's padded plane stride
+            ps = _lib.query("svbrdf_plane_stride", r.res)
+            moments = torch.zeros((2, 9 * ps), dtype=torch.float32, device=self.device)
+            prepared = _planes(torch.zeros(3 * r.n_of_imgs * ps, dtype=torch.float32, device=self.device),
+                               tg.shape, ps)
+            prepared.copy_(dense)
             self.renderers.append(r)
             self.textures.append(tex)
-            self.exp_avg.append(torch.zeros_like(tex))
-            self.exp_avg_sq.append(torch.zeros_like(tex))
+            self.plane_strides.append(ps)
+            self.exp_avg.append(_planes(moments[0], tex.shape, ps))
+            self.exp_avg_sq.append(_planes(moments[1], tex.shape, ps))
             self.targets.append(prepared)
         self.light_pow = [r.light_pow.detach().to(self.device, torch.float32).reshape(3).contiguous()
                           for r in self.renderers]
@@ -113,7 +121,7 @@ class BatchedStep:
             table[i] = Material(self.textures[i].data_ptr(), self.exp_avg[i].data_ptr(),
                                 self.exp_avg_sq[i].data_ptr(), self.targets[i].data_ptr(),
                                 r.camera_pos.data_ptr(), r.light_pos.data_ptr(), self.light_pow[i].data_ptr(),
-                                r.res, r.n_of_imgs, r.size, 0, 0, 0)
+                                r.res, r.n_of_imgs, r.size, 0, 0, self.plane_strides[i])
         n_tiles = _lib.query("svbrdf_batch_tile_count", table, self.n, tile)
         if n_tiles == 0:
             raise ValueError("invalid batch for tile planning")

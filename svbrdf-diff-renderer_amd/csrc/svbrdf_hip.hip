@@ -37,6 +37,19 @@ constexpr int kMaxViews = 1024;
 using svbrdf_internal::check_launch;
 using svbrdf_internal::fail;
 
+// Plane stride of the fused step's buffers: 0 selects res^2; otherwise at
+// least res^2, and the 9 planes of one tensor must stay inside one 2 GiB
+// buffer range (offsets are 32-bit).
+int check_plane_stride(const char *what, int res, size_t plane_stride, unsigned &ps) {
+    const size_t npix = (size_t)res * res;
+    const size_t v = plane_stride ? plane_stride : npix;
+    if (v < npix) return fail(SVBRDF_EINVAL, "%s: plane_stride=%zu < res^2=%zu", what, v, npix);
+    if (v * 9u * 4u > 0x7fffffffu)
+        return fail(SVBRDF_ESHAPE, "%s: plane_stride=%zu puts 9 planes beyond 2 GiB", what, v);
+    ps = (unsigned)v;
+    return SVBRDF_OK;
+}
+
 int check_common(const char *what, const void *tex, int res, int n_views, const void *cam,
                  const void *light, const void *pow_) {
     if (!tex || !cam || !light || !pow_) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
@@ -112,6 +125,14 @@ __device__ __forceinline__ Tensor tensor(const float *base, unsigned n_planes, u
     t.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), 0, (int)(n_planes * t.stride), 0x00020000);
     return t;
 }
+// Planes `stride` floats apart (>= the pixel count): buffers the caller lays
+// out with svbrdf_plane_stride() padding between planes.
+__device__ __forceinline__ Tensor tensor_ps(const float *base, unsigned n_planes, unsigned stride) {
+    Tensor t;
+    t.stride = stride * 4u;
+    t.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), 0, (int)(n_planes * t.stride), 0x00020000);
+    return t;
+}
 __device__ __forceinline__ float ld(const Tensor &t, unsigned c, unsigned byte_off) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(t.r, byte_off, (int)(c * t.stride), 0));
 }
@@ -171,12 +192,20 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
                                               float *__restrict__ out) {
     Maps m[NPIX];
     const Tensor T = tensor(tex, 9, npix);
+    // co-located: img_c = (P_c f1_c + DGq P_c F_c) (n.l)/d^2, the view-
+    // independent products formed once per pixel
+    float Pf1[NPIX][3], PF[NPIX][3];
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
         float t[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) t[c] = ld(T, c, px[q].off);
         decode(t, m[q]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Pf1[q][c] = P[c] * m[q].f1[c];
+            PF[q][c] = P[c] * m[q].Fc[c];
+        }
     }
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
@@ -185,10 +214,10 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
         for (int q = 0; q < NPIX; ++q) {
             float img[3];
             if constexpr (COLOC) {
-                ShadeC s;
-                shade_coloc(m[q], px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, s);
+                float DGq, nl_d2;
+                shade_coloc_fwd(m[q], px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, DGq, nl_d2);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) img[c] = P[c] * (m[q].f1[c] + s.DGq * m[q].Fc[c]) * s.nl_d2;
+                for (int c = 0; c < 3; ++c) img[c] = (Pf1[q][c] + DGq * PF[q][c]) * nl_d2;
             } else {
                 Geom g;
                 geometry<false>(px[q].px, px[q].py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
@@ -200,7 +229,7 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
             if (px[q].valid) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float ic = tclamp(img[c], kEps, 1.0f);
+                    const float ic = tclamp_med3(img[c], kEps, 1.0f);
                     st(O, c, px[q].off, fexp2(flog2(ic) * kInvGamma));
                 }
             }
@@ -441,13 +470,13 @@ __device__ __forceinline__ float loss_channel(float img, float y, float K2, floa
 // without K2 and scales them once per pixel.
 template <int LOSS, bool OPT_LIGHT, int UNROLL>
 __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, const float *__restrict__ tgt,
-                                                  unsigned npix, int n_views, const float *__restrict__ cam,
+                                                  unsigned ps, int n_views, const float *__restrict__ cam,
                                                   const float P[3], float K2, float &loss, float (&rP)[3],
                                                   Accum &a) {
     float yn[3];
     if (SVBRDF_TARGET_PREFETCH) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) yn[c] = ld(tensor(tgt, 3, npix), c, px.off);
+        for (int c = 0; c < 3; ++c) yn[c] = ld(tensor_ps(tgt, 3, ps), c, px.off);
     }
 #pragma unroll UNROLL
     for (int k = 0; k < n_views; ++k) {
@@ -456,14 +485,14 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
         if (SVBRDF_TARGET_PREFETCH) {
             // issue view k+1's loads now (the last view re-reads its own, an L1/L2 hit)
             const int kn = k + 1 < n_views ? k + 1 : k;
-            const Tensor TN = tensor(tgt + (size_t)(3 * kn) * npix, 3, npix);
+            const Tensor TN = tensor_ps(tgt + (size_t)(3 * kn) * ps, 3, ps);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 y[c] = yn[c];
                 yn[c] = ld(TN, c, px.off);
             }
         } else {
-            const Tensor TK = tensor(tgt + (size_t)(3 * k) * npix, 3, npix);
+            const Tensor TK = tensor_ps(tgt + (size_t)(3 * k) * ps, 3, ps);
 #pragma unroll
             for (int c = 0; c < 3; ++c) y[c] = ld(TK, c, px.off);
         }
@@ -509,11 +538,11 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
 template <int LOSS, bool OPT_LIGHT, bool COLOC, int NPIX, int UNROLL>
 __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__restrict__ tex,
                                              float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
-                                             const float *__restrict__ tgt, unsigned npix, int n_views,
-                                             const float *__restrict__ cam, const float *__restrict__ light,
-                                             const float P[3], float K2, const StepScalars &sc, float &loss,
-                                             float (&gP)[3]) {
-    const Tensor TX = tensor(tex, 9, npix), TM = tensor(exp_avg, 9, npix), TV = tensor(exp_avg_sq, 9, npix);
+                                             const float *__restrict__ tgt, unsigned tps, unsigned ps,
+                                             int n_views, const float *__restrict__ cam,
+                                             const float *__restrict__ light, const float P[3], float K2,
+                                             const StepScalars &sc, float &loss, float (&gP)[3]) {
+    const Tensor TX = tensor_ps(tex, 9, tps), TM = tensor_ps(exp_avg, 9, ps), TV = tensor_ps(exp_avg_sq, 9, ps);
     float traw[NPIX][9];
     float mm[NPIX][9], vv[NPIX][9];
     Maps m[NPIX];
@@ -542,7 +571,7 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
     if constexpr (COLOC) {
 #pragma unroll
         for (int q = 0; q < NPIX; ++q)
-            fused_views_coloc<LOSS, OPT_LIGHT, UNROLL>(px[q], m[q], tgt, npix, n_views, cam, P, K2, lq[q], rP[q],
+            fused_views_coloc<LOSS, OPT_LIGHT, UNROLL>(px[q], m[q], tgt, ps, n_views, cam, P, K2, lq[q], rP[q],
                                                        a[q]);
     } else {
         float yn[NPIX][3];
@@ -550,13 +579,13 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
 #pragma unroll
             for (int q = 0; q < NPIX; ++q)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) yn[q][c] = ld(tensor(tgt, 3, npix), c, px[q].off);
+                for (int c = 0; c < 3; ++c) yn[q][c] = ld(tensor_ps(tgt, 3, ps), c, px[q].off);
         }
 #pragma unroll UNROLL
         for (int k = 0; k < n_views; ++k) {
             const ViewPos vp = load_view(cam, light, k);
-            const Tensor TK = tensor(tgt + (size_t)(3 * k) * npix, 3, npix);
-            const Tensor TN = tensor(tgt + (size_t)(3 * (k + 1 < n_views ? k + 1 : k)) * npix, 3, npix);
+            const Tensor TK = tensor_ps(tgt + (size_t)(3 * k) * ps, 3, ps);
+            const Tensor TN = tensor_ps(tgt + (size_t)(3 * (k + 1 < n_views ? k + 1 : k)) * ps, 3, ps);
 #pragma unroll
             for (int q = 0; q < NPIX; ++q) {
                 float y[3];
@@ -583,8 +612,9 @@ __device__ __forceinline__ void fused_pixels(const Pix (&px)[NPIX], float *__res
 #pragma unroll
     for (int q = 0; q < NPIX; ++q) {
         if (!px[q].valid) continue;
-        // parameters that are NaN make the loss NaN, as in the reference
-        bool finite = true;
+        // parameters or a light power that are NaN make the loss NaN, as in
+        // the reference
+        bool finite = P[0] == P[0] && P[1] == P[1] && P[2] == P[2];
 #pragma unroll
         for (int c = 0; c < 9; ++c) finite = finite && (traw[q][c] == traw[q][c]);
         loss += finite ? lq[q] : __builtin_nanf("");
@@ -722,7 +752,8 @@ __device__ __forceinline__ void advance_step(svbrdf_step_state *st, float step, 
 template <int LOSS, bool OPT_LIGHT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_FUSED_WAVES))) void fused_step_kernel(
     float *__restrict__ tex, float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
-    const float *__restrict__ tgt, int res, int n_views, float size, const float *__restrict__ cam,
+    const float *__restrict__ tgt, int res, unsigned tps, unsigned ps, int n_views, float size,
+    const float *__restrict__ cam,
     const float *__restrict__ light, float *__restrict__ lpow, double lr, double beta1, double beta2,
     double eps, svbrdf_step_state *st, void *ws, float *loss_history, int history_len) {
     constexpr int NPIX = SVBRDF_FUSED_NPIX;
@@ -748,10 +779,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     float loss = 0.f;
     float gPs[3] = {0.f, 0.f, 0.f};
     if (all_colocated(cam, light, n_views))
-        fused_pixels<LOSS, OPT_LIGHT, true, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, npix,
+        fused_pixels<LOSS, OPT_LIGHT, true, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, tps, ps,
                                                                         n_views, cam, light, P, K2, sc, loss, gPs);
     else
-        fused_pixels<LOSS, OPT_LIGHT, false, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, npix,
+        fused_pixels<LOSS, OPT_LIGHT, false, NPIX, SVBRDF_FUSED_UNROLL>(px, tex, exp_avg, exp_avg_sq, tgt, tps, ps,
                                                                          n_views, cam, light, P, K2, sc, loss, gPs);
     const float bl = block_sum(loss, lds);
     float bp[3] = {0.f, 0.f, 0.f};
@@ -854,6 +885,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
         const svbrdf_material mt = mats[tile_mat[id]];
         const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
         const unsigned first = (id - mt.tile_begin) * (unsigned)tile_pixels;
+        const unsigned ps = mt.plane_stride ? mt.plane_stride : npix;
         const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
         const float P[3] = {ldu(mt.light_pow, 0), ldu(mt.light_pow, 1), ldu(mt.light_pow, 2)};
         const bool coloc = all_colocated(mt.camera_pos, mt.light_pos, mt.n_views);
@@ -866,11 +898,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
                 px[q] = pixel_at(first + base + q * kBlock + threadIdx.x, npix, mt.res, mt.size);
             if (coloc)
                 fused_pixels<LOSS, false, true, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
-                                                              mt.targets_prepared, npix, mt.n_views, mt.camera_pos,
+                                                              mt.targets_prepared, npix, ps, mt.n_views, mt.camera_pos,
                                                               mt.light_pos, P, K2, sc, loss, gPs);
             else
                 fused_pixels<LOSS, false, false, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
-                                                               mt.targets_prepared, npix, mt.n_views, mt.camera_pos,
+                                                               mt.targets_prepared, npix, ps, mt.n_views, mt.camera_pos,
                                                                mt.light_pos, P, K2, sc, loss, gPs);
         }
         // block_sum ends with a barrier, so every thread has read s_slot before
@@ -1051,13 +1083,20 @@ int svbrdf_step_state_init(void *state, double beta1, double beta2, void *stream
     return check_launch("svbrdf_step_state_init");
 }
 
+size_t svbrdf_plane_stride(int res) {
+    if (res < 1 || res > kMaxRes) return 0;
+    const size_t npix = (size_t)res * res;
+    return (npix + 63) / 64 * 64 + 448;
+}
+
 size_t svbrdf_fused_step_workspace_bytes(int res) {
     if (res < 1 || res > kMaxRes) return 0;
     return fused_workspace_bytes(n_blocks(res, SVBRDF_FUSED_NPIX));
 }
 
 int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const float *targets_prepared, int res,
-                      int n_views, float size, const float *camera_pos, const float *light_pos, float *light_pow,
+                      size_t texture_stride, size_t plane_stride, int n_views, float size,
+                      const float *camera_pos, const float *light_pos, float *light_pow,
                       int loss_mode, int optim_light, double lr, double beta1, double beta2, double eps,
                       void *state, float *loss_history, int history_len, void *workspace,
                       size_t workspace_bytes, void *stream) {
@@ -1074,13 +1113,16 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
         return fail(SVBRDF_ESPACE, "svbrdf_fused_step: workspace of %zu bytes < %zu needed", workspace_bytes, need);
     if (((uintptr_t)workspace & 255u) || ((uintptr_t)state & 15u))
         return fail(SVBRDF_EINVAL, "svbrdf_fused_step: workspace must be 256-byte and state 16-byte aligned");
+    unsigned tps = 0, ps = 0;
+    if ((rc = check_plane_stride("svbrdf_fused_step", res, texture_stride, tps))) return rc;
+    if ((rc = check_plane_stride("svbrdf_fused_step", res, plane_stride, ps))) return rc;
     const unsigned nb = n_blocks(res, SVBRDF_FUSED_NPIX);
     hipStream_t s = (hipStream_t)stream;
     svbrdf_step_state *st = (svbrdf_step_state *)state;
 #define SVBRDF_LAUNCH_STEP(L, O)                                                                             \
     hipLaunchKernelGGL((fused_step_kernel<L, O>), dim3(nb), dim3(kBlock), 0, s, textures, exp_avg, exp_avg_sq, \
-                       targets_prepared, res, n_views, size, camera_pos, light_pos, light_pow, lr, beta1,      \
-                       beta2, eps, st, workspace, loss_history, history_len)
+                       targets_prepared, res, tps, ps, n_views, size, camera_pos, light_pos, light_pow, lr,    \
+                       beta1, beta2, eps, st, workspace, loss_history, history_len)
     if (loss_mode == SVBRDF_LOSS_LOG_L2) {
         if (optim_light) SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, true);
         else SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, false);
@@ -1143,6 +1185,8 @@ int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixe
             return fail(SVBRDF_EINVAL, "svbrdf_batch_plan: material %d has a null pointer", i);
         if (m.res < 1 || m.res > kMaxRes || m.n_views < 1 || m.n_views > kMaxViews)
             return fail(SVBRDF_ESHAPE, "svbrdf_batch_plan: material %d has res=%d n_views=%d", i, m.res, m.n_views);
+        unsigned ps = 0;
+        if (int rc = check_plane_stride("svbrdf_batch_plan", m.res, m.plane_stride, ps)) return rc;
     }
     const size_t need = svbrdf_batch_tile_count(materials, n_materials, tile_pixels);
     if (need != n_tiles)

@@ -42,6 +42,18 @@ def loss_mode(loss):
         raise ValueError(f"unknown loss {loss!r}; expected one of {sorted(LOSSES)}") from None
 
 
+def _planes(buf, shape, stride):
+    """[.., P, R, R] view of a flat buffer whose planes start `stride` floats
+    apart (leading dimensions are plane groups, contiguous in P planes)."""
+    shape = tuple(int(x) for x in shape)
+    strides = [shape[-1], 1]
+    acc = stride
+    for d in reversed(shape[:-2]):
+        strides.insert(0, acc)
+        acc *= d
+    return buf.as_strided(shape, strides)
+
+
 def epoch_segments(epochs, save_every=100, sync_every=0, saving=True):
     """Splits an optimisation of `epochs` iterations at the epochs after which
     the host acts: [(first, last, save)], 0-based and inclusive.  `save` marks
@@ -70,7 +82,7 @@ class FusedStep:
     """
 
     def __init__(self, renderer, textures, targets, lr, loss="l2", betas=(0.9, 0.999), eps=1e-8,
-                 optim_light=False, history=0):
+                 optim_light=False, history=0, plane_stride=None):
         self.renderer = renderer
         self.res, self.n_views = renderer.res, renderer.n_of_imgs
         dev = renderer.device
@@ -94,12 +106,26 @@ class FusedStep:
             self.light_pow = lp
         else:
             self.light_pow = renderer.light_pow.detach().to(torch.float32).reshape(3).contiguous()
-        self.exp_avg = torch.zeros_like(textures)
-        self.exp_avg_sq = torch.zeros_like(textures)
-        self.targets = torch.empty_like(targets, dtype=torch.float32, device=dev).contiguous()
+        # Adam's moments and the prepared targets belong to this object: they
+        # are laid out with the library's padded plane stride (planes 4 MiB
+        # apart at 1024^2 would share HBM channels) unless plane_stride says
+        # otherwise (0: dense); the textures are the caller's dense tensor,
+        # updated in place.
+        self.texture_stride = 0
+        if plane_stride is None:
+            plane_stride = _lib.query("svbrdf_plane_stride", self.res)
+        self.plane_stride = int(plane_stride)
+        ps = self.plane_stride or self.res * self.res
+        self._moments = torch.zeros((2, 9 * ps), dtype=torch.float32, device=dev)
+        self.exp_avg = _planes(self._moments[0], textures.shape, ps)
+        self.exp_avg_sq = _planes(self._moments[1], textures.shape, ps)
+        self._targets = torch.zeros(3 * self.n_views * ps, dtype=torch.float32, device=dev)
+        self.targets = _planes(self._targets, (self.n_views, 3, self.res, self.res), ps)
         tg = targets.detach().to(device=dev, dtype=torch.float32).contiguous()
         s = _lib.stream_handle(dev)
-        _lib.call("svbrdf_prepare_targets", _lib.ptr(tg), tg.numel(), self.mode, _lib.ptr(self.targets), s)
+        dense = torch.empty_like(tg)
+        _lib.call("svbrdf_prepare_targets", _lib.ptr(tg), tg.numel(), self.mode, _lib.ptr(dense), s)
+        self.targets.copy_(dense)
         nstate = _lib.query("svbrdf_step_state_bytes")
         self.state = torch.zeros((nstate + 15) // 16 * 4, dtype=torch.float32, device=dev)
         nws = _lib.query("svbrdf_fused_step_workspace_bytes", self.res)
@@ -113,16 +139,19 @@ class FusedStep:
         self.graph_steps = 0
         self._graphs = {}
 
-    def step(self):
-        """Launch one fused iteration on the current stream (asynchronous)."""
+    def _launch(self, stream):
         r = self.renderer
         hist = self.history if self.history.numel() else None
         _lib.call("svbrdf_fused_step", _lib.ptr(self.textures), _lib.ptr(self.exp_avg), _lib.ptr(self.exp_avg_sq),
-                  _lib.ptr(self.targets), self.res, self.n_views, r.size, _lib.ptr(r.camera_pos),
-                  _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode, int(self.optim_light), self.lr,
-                  self.beta1, self.beta2, self.eps, _lib.ptr(self.state), _lib.ptr(hist),
-                  self.history.numel(), _lib.ptr(self.workspace), self.workspace.numel() * 4,
-                  _lib.stream_handle(self.textures.device))
+                  _lib.ptr(self.targets), self.res, self.texture_stride, self.plane_stride, self.n_views, r.size,
+                  _lib.ptr(r.camera_pos), _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode,
+                  int(self.optim_light), self.lr, self.beta1, self.beta2, self.eps, _lib.ptr(self.state),
+                  _lib.ptr(hist), self.history.numel(), _lib.ptr(self.workspace), self.workspace.numel() * 4,
+                  stream)
+
+    def step(self):
+        """Launch one fused iteration on the current stream (asynchronous)."""
+        self._launch(_lib.stream_handle(self.textures.device))
         self.steps_launched += 1
 
     def capture(self, steps=1):
@@ -161,15 +190,8 @@ class FusedStep:
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             g.capture_begin()
-            r = self.renderer
-            hist = self.history if self.history.numel() else None
             for _ in range(steps):
-                _lib.call("svbrdf_fused_step", _lib.ptr(self.textures), _lib.ptr(self.exp_avg),
-                          _lib.ptr(self.exp_avg_sq), _lib.ptr(self.targets), self.res, self.n_views, r.size,
-                          _lib.ptr(r.camera_pos), _lib.ptr(r.light_pos), _lib.ptr(self.light_pow), self.mode,
-                          int(self.optim_light), self.lr, self.beta1, self.beta2, self.eps,
-                          _lib.ptr(self.state), _lib.ptr(hist), self.history.numel(),
-                          _lib.ptr(self.workspace), self.workspace.numel() * 4, ctypes.c_void_p(s.cuda_stream))
+                self._launch(ctypes.c_void_p(s.cuda_stream))
             g.capture_end()
         torch.cuda.current_stream().wait_stream(s)
         return g

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 2
+    assert lib.svbrdf_abi_version() == 3
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
@@ -47,6 +47,13 @@ def test_queries(lib, pkg):
     assert lib.svbrdf_fused_step_workspace_bytes(1) == 8448 + 2 * 256
     assert lib.svbrdf_fused_step_workspace_bytes(0) == 0
     assert lib.svbrdf_render_backward_workspace_bytes(13) == 1 * 16
+    # padded plane stride: res^2 rounded up to 64 floats, plus 448
+    assert lib.svbrdf_plane_stride(1024) == 1024 * 1024 + 448
+    assert lib.svbrdf_plane_stride(1023) == 1046592 + 448 and 1046592 >= 1023 * 1023
+    assert lib.svbrdf_plane_stride(1) == 64 + 448
+    assert lib.svbrdf_plane_stride(0) == 0 and lib.svbrdf_plane_stride(lib.svbrdf_max_res() + 1) == 0
+    # 9 planes at the largest resolution stay inside one 2 GiB buffer range
+    assert 9 * 4 * lib.svbrdf_plane_stride(lib.svbrdf_max_res()) < 2 ** 31
 
 
 def _fake(n=1):
@@ -74,11 +81,13 @@ def test_backward_requires_workspace_for_light_gradient(lib):
 
 
 def _step(lib, **kw):
-    a = dict(tex=_fake(1), m=_fake(2), v=_fake(3), tgt=_fake(4), res=64, views=9, cam=_fake(5), light=_fake(6),
+    a = dict(tex=_fake(1), m=_fake(2), v=_fake(3), tgt=_fake(4), res=64, tps=0, ps=0, views=9, cam=_fake(5),
+             light=_fake(6),
              pow_=_fake(7), mode=1, light_opt=0, lr=0.01, b1=0.9, b2=0.999, eps=1e-8, state=_fake(8),
              hist=None, hlen=0, ws=_fake(9), wsb=lib.svbrdf_fused_step_workspace_bytes(64))
     a.update(kw)
-    return lib.svbrdf_fused_step(a["tex"], a["m"], a["v"], a["tgt"], a["res"], a["views"], ctypes.c_float(17.0),
+    return lib.svbrdf_fused_step(a["tex"], a["m"], a["v"], a["tgt"], a["res"], a["tps"], a["ps"], a["views"],
+                                 ctypes.c_float(17.0),
                                  a["cam"], a["light"], a["pow_"], a["mode"], a["light_opt"],
                                  ctypes.c_double(a["lr"]), ctypes.c_double(a["b1"]), ctypes.c_double(a["b2"]),
                                  ctypes.c_double(a["eps"]), a["state"], a["hist"], a["hlen"], a["ws"], a["wsb"], None)
@@ -93,6 +102,10 @@ def test_fused_step_validation(lib):
     assert _step(lib, ws=ctypes.c_void_p(0x1004)) == -1      # misaligned workspace
     assert _step(lib, ws=ctypes.c_void_p(0x1080)) == -1      # only 128-byte aligned
     assert _step(lib, res=0) == -2
+    assert _step(lib, ps=64 * 64 - 1) == -1                  # planes overlap
+    assert _step(lib, tps=100) == -1
+    assert _step(lib, ps=60_000_000) == -2                   # 9 planes beyond 2 GiB
+    assert b"plane_stride" in lib.svbrdf_last_error()
 
 
 def test_flush_validation(lib):

@@ -18,12 +18,12 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 
 
-def make_problem(pkg, tex0, target, res, size, cam, light, P, loss, lr=0.01, optim_light=False, history=64):
+def make_problem(pkg, tex0, target, res, size, cam, light, P, loss, lr=0.01, optim_light=False, history=64, **kw):
     cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
     r = pkg.Microfacet(res, cam.shape[0], size, cl, DEV)
     tex = torch.tensor(tex0, device=DEV).reshape(1, 9, res, res).contiguous()
     tg = torch.tensor(target, device=DEV)
-    return pkg.FusedStep(r, tex, tg, lr, loss=loss, optim_light=optim_light, history=history), tex
+    return pkg.FusedStep(r, tex, tg, lr, loss=loss, optim_light=optim_light, history=history, **kw), tex
 
 
 def eager_loss_grad(tex, target, res, size, cam, light, P, mode, dtype):
@@ -122,6 +122,28 @@ def test_graph_replay_matches_eager_steps(pkg):
     np.testing.assert_array_equal(fa.exp_avg_sq.cpu().numpy(), fb.exp_avg_sq.cpu().numpy())
     np.testing.assert_array_equal(fa.losses(), fb.losses())
     assert fb.read_state()["step"] == 3.0
+
+
+@pytest.mark.parametrize("res,V,coloc,light", [(33, 5, False, False), (128, 9, True, False), (64, 9, True, True)])
+def test_plane_layouts_are_bit_identical(pkg, res, V, coloc, light):
+    """Moments and targets at the padded plane stride, dense (0) or an odd
+    stride: the same arithmetic on the same values."""
+    gt, _, size, cam, light_pos, P = scene(pkg, res, V, seed=6, coloc=coloc)
+    init = pkg.synthetic.init_textures(gt, seed=2)
+    target = M.render(gt, res, size, cam, light_pos, P, np.float32)
+    runs = []
+    for ps in (None, 0, res * res + 37):
+        fs, tex = make_problem(pkg, init, target, res, size, cam, light_pos, P, "log_l2", optim_light=light,
+                               plane_stride=ps)
+        assert fs.plane_stride == (pkg._lib.query("svbrdf_plane_stride", res) if ps is None else ps)
+        for _ in range(4):
+            fs.step()
+        torch.cuda.synchronize()
+        runs.append((tex.cpu().numpy(), fs.exp_avg.cpu().numpy(), fs.exp_avg_sq.cpu().numpy(), fs.losses(),
+                     fs.light_pow.cpu().numpy()))
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_deterministic(pkg):
