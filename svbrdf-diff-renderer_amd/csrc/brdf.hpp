@@ -309,12 +309,14 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     s.inv_d2 = frcp(sv);
     s.nv_raw = m.n[0] * s.v[0] + m.n[1] * s.v[1] + m.n[2] * s.v[2];
     s.nv = tmax0(s.nv_raw);
-    // GGX with 1 - (n.h)^2 = |n x v|^2 (no cancellation at the highlight)
+    // GGX with 1 - (n.h)^2 = |v - (n.v) n|^2 for unit n, v: the components
+    // are O(sin) with O(ulp) absolute error, so there is no cancellation at
+    // the highlight (1 - c^2 itself would lose all digits as c -> 1)
     s.c2 = s.nv * s.nv;
-    const float xx = m.n[1] * s.v[2] - m.n[2] * s.v[1];
-    const float xy = m.n[2] * s.v[0] - m.n[0] * s.v[2];
-    const float xz = m.n[0] * s.v[1] - m.n[1] * s.v[0];
-    const float one_m_c2 = s.nv_raw < 0.0f ? 1.0f : xx * xx + xy * xy + xz * xz;
+    const float wx = s.v[0] - s.nv_raw * m.n[0];
+    const float wy = s.v[1] - s.nv_raw * m.n[1];
+    const float wz = s.v[2] - s.nv_raw * m.n[2];
+    const float one_m_c2 = s.nv_raw < 0.0f ? 1.0f : wx * wx + wy * wy + wz * wz;
     s.den = s.c2 * m.a2 + one_m_c2;
     s.iqd = frcp(kPi * (s.den * s.den) + kEps);
     s.D = m.a2 * s.iqd;
@@ -343,10 +345,8 @@ __device__ __forceinline__ void shade_coloc_fwd(const Maps &m, float px, float p
     const float nv_raw = m.n[0] * v0 + m.n[1] * v1 + m.n[2] * v2;
     const float nv = tmax0(nv_raw);
     const float c2 = nv * nv;
-    const float xx = m.n[1] * v2 - m.n[2] * v1;
-    const float xy = m.n[2] * v0 - m.n[0] * v2;
-    const float xz = m.n[0] * v1 - m.n[1] * v0;
-    const float one_m_c2 = nv_raw < 0.0f ? 1.0f : xx * xx + xy * xy + xz * xz;
+    const float wx = v0 - nv_raw * m.n[0], wy = v1 - nv_raw * m.n[1], wz = v2 - nv_raw * m.n[2];
+    const float one_m_c2 = nv_raw < 0.0f ? 1.0f : wx * wx + wy * wy + wz * wz;
     const float den = c2 * m.a2 + one_m_c2;
     const float sd = nv * m.omk + m.k;
     const float q = (kPi * (den * den) + kEps) * (sd * sd) * (4.0f * c2 + kEps);
@@ -356,15 +356,17 @@ __device__ __forceinline__ void shade_coloc_fwd(const Maps &m, float px, float p
 
 // Backward of shade_coloc given dL/df_c (masked) and the direct dL/d(n.l) of
 // img = P f (n.l) / d^2.  Accumulates into `a`; a.T collects dL/dF_c, to be
-// scaled by dF/dspe = 1 - 2^(A+B) once per pixel (coloc_finish).
+// scaled by dF/dspe = 1 - 2^(A+B) once per pixel (coloc_finish).  Fw are the
+// Fresnel factors paired with gf: F_c, or P_c F_c when the caller passes
+// gf / P_c (and scales a.S and a.T by P_c after the view loop).
 __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, const float gf[3], float gnl,
-                                               float rnv, Accum &a) {
+                                               float rnv, const float Fw[3], Accum &a) {
     float W;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         a.S[c] += gf[c];
         a.T[c] += gf[c] * s.DGq;
-        W = c == 0 ? gf[c] * m.Fc[c] : W + gf[c] * m.Fc[c];
+        W = c == 0 ? gf[c] * Fw[c] : W + gf[c] * Fw[c];
     }
     // f2 = D G F / q with D, G, q functions of n.v (= n.l = n.h) and W =
     // sum_c dL/df_c F_c; WD = W D G / q.  With rnv = 1/n.v (also used for

@@ -292,7 +292,7 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
                 gnl += gimg * P[c] * f;
                 if (LIGHT_GRAD) a.gP[c] += gimg * f * s.nl_d2;
             }
-            backward_coloc(m, s, gf, gnl * s.inv_d2, frcp(fmaxf(s.nv, 1e-30f)), a);
+            backward_coloc(m, s, gf, gnl * s.inv_d2, frcp(fmaxf(s.nv, 1e-30f)), m.Fc, a);
         } else {
             Geom g;
             geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
@@ -478,6 +478,15 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
 #pragma unroll
         for (int c = 0; c < 3; ++c) yn[c] = ld(tensor_ps(tgt, 3, ps), c, px.off);
     }
+    // P_c f_c = P_c f1_c + DGq P_c F_c: the light power folded into the
+    // view-independent factors.  The loop then accumulates dL/d(P_c f_c)
+    // (= dL/df_c / P_c) and P_c is applied once with K2 after the loop.
+    float Pf1[3], PF[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Pf1[c] = P[c] * m.f1[c];
+        PF[c] = P[c] * m.Fc[c];
+    }
 #pragma unroll UNROLL
     for (int k = 0; k < n_views; ++k) {
         const float cx = ldu(cam, 3 * k + 0), cy = ldu(cam, 3 * k + 1), cz = ldu(cam, 3 * k + 2);
@@ -501,8 +510,8 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
         float gf[3], rsum;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float f = m.f1[c] + s.DGq * m.Fc[c];
-            const float img = P[c] * f * s.nl_d2;
+            const float pf = Pf1[c] + s.DGq * PF[c];
+            const float img = pf * s.nl_d2;
             const float ic = fminf(fmaxf(img, kEps), 1.0f);
             const bool inside = (ic == img);
             const float lic = flog2(ic);
@@ -516,20 +525,20 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
                 rm = inside ? r * o : 0.0f;
             }
             loss += r * r;
-            // dL/df_c / K2: every accumulator is linear in these, so K2 is
-            // applied once per pixel after the loop
-            gf[c] = rm * frcp(fmaxf(f, 1e-30f));
+            // dL/d(P_c f_c) / K2: every accumulator is linear in these, so
+            // K2 (and P_c) is applied once per pixel after the loop
+            gf[c] = rm * frcp(fmaxf(pf, 1e-30f));
             rsum = c == 0 ? rm : rsum + rm;   // no "0 + x": it is not x for x = -0
             if (OPT_LIGHT) rP[c] += rm;
         }
         const float rnv = frcp(fmaxf(s.nv, 1e-30f));
-        backward_coloc(m, s, gf, rsum * rnv, rnv, a);
+        backward_coloc(m, s, gf, rsum * rnv, rnv, PF, a);
     }
     coloc_finish(a);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        a.S[c] *= K2;
-        a.T[c] *= K2;
+        a.S[c] *= K2 * P[c];
+        a.T[c] *= K2 * P[c];
         a.gn[c] *= K2;
     }
     a.A *= K2;

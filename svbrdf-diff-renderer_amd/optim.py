@@ -75,14 +75,16 @@ def epoch_segments(epochs, save_every=100, sync_every=0, saving=True):
 class FusedStep:
     """One optimisation problem: textures (updated in place) against targets.
 
-    textures: [1,9,R,R] or [9,R,R] float32 CUDA tensor, updated in place.
+    textures: [1,9,R,R] or [9,R,R] float32 CUDA tensor, updated in place
+              (with own_textures=True the step works on a padded copy,
+              ``self.textures``, and write_back() updates this tensor).
     targets:  [V,3,R,R] float32 images in gamma space (what Microfacet.eval
               produces and what src/svbrdf.py:191-204 loads).
     renderer: a :class:`Microfacet` (geometry, light power, size).
     """
 
     def __init__(self, renderer, textures, targets, lr, loss="l2", betas=(0.9, 0.999), eps=1e-8,
-                 optim_light=False, history=0, plane_stride=None):
+                 optim_light=False, history=0, plane_stride=None, own_textures=False):
         self.renderer = renderer
         self.res, self.n_views = renderer.res, renderer.n_of_imgs
         dev = renderer.device
@@ -109,13 +111,22 @@ class FusedStep:
         # Adam's moments and the prepared targets belong to this object: they
         # are laid out with the library's padded plane stride (planes 4 MiB
         # apart at 1024^2 would share HBM channels) unless plane_stride says
-        # otherwise (0: dense); the textures are the caller's dense tensor,
-        # updated in place.
-        self.texture_stride = 0
+        # otherwise (0: dense).  The textures are the caller's dense tensor,
+        # updated in place, or with own_textures=True a copy at the same
+        # padded stride that write_back() returns to the caller's tensor.
         if plane_stride is None:
             plane_stride = _lib.query("svbrdf_plane_stride", self.res)
         self.plane_stride = int(plane_stride)
         ps = self.plane_stride or self.res * self.res
+        self.source = textures
+        if own_textures:
+            self._tex = torch.zeros(9 * ps, dtype=torch.float32, device=dev)
+            self.textures = _planes(self._tex, textures.shape, ps)
+            with torch.no_grad():
+                self.textures.copy_(textures)
+            self.texture_stride = self.plane_stride
+        else:
+            self.texture_stride = 0
         self._moments = torch.zeros((2, 9 * ps), dtype=torch.float32, device=dev)
         self.exp_avg = _planes(self._moments[0], textures.shape, ps)
         self.exp_avg_sq = _planes(self._moments[1], textures.shape, ps)
@@ -199,6 +210,13 @@ class FusedStep:
     def replay(self):
         self.graph.replay()
         self.steps_launched += self.graph_steps
+
+    def write_back(self):
+        """Copies the current maps into the tensor given at construction when
+        this step keeps its own copy (own_textures=True); asynchronous."""
+        if self.textures is not self.source:
+            with torch.no_grad():
+                self.source.copy_(self.textures)
 
     def flush(self):
         """Records the last step's loss if it is still pending (asynchronous).
@@ -317,8 +335,10 @@ class SvbrdfOptim:
             r.update_light(lp)
         with torch.no_grad():
             tex = self.textures.detach()
+        # the step keeps its own padded copy of the maps; self.textures is
+        # brought up to date after every stretch of epochs
         self.fused = FusedStep(r, tex, self.targets, lr, loss=self.loss, optim_light=optim_light,
-                               history=epochs)
+                               history=epochs, own_textures=True)
         tmp_dir = None
         if svbrdf_obj is not None and hasattr(svbrdf_obj, "optimize_dir"):
             stamp = str(datetime.datetime.now()).replace(" ", "-").replace(":", "-").replace(".", "-")
@@ -326,6 +346,7 @@ class SvbrdfOptim:
             tmp_dir.mkdir(parents=True, exist_ok=True)
         for first, last, save in epoch_segments(epochs, save_every, sync_every, tmp_dir is not None):
             self.fused.run(last - first + 1, graph_steps)
+            self.fused.write_back()
             if sync_every:
                 torch.cuda.synchronize(self.device)
             if save:

@@ -127,20 +127,26 @@ def test_graph_replay_matches_eager_steps(pkg):
 @pytest.mark.parametrize("res,V,coloc,light", [(33, 5, False, False), (128, 9, True, False), (64, 9, True, True)])
 def test_plane_layouts_are_bit_identical(pkg, res, V, coloc, light):
     """Moments and targets at the padded plane stride, dense (0) or an odd
-    stride: the same arithmetic on the same values."""
+    stride, and the maps in the caller's tensor or in the step's own padded
+    copy: the same arithmetic on the same values."""
     gt, _, size, cam, light_pos, P = scene(pkg, res, V, seed=6, coloc=coloc)
     init = pkg.synthetic.init_textures(gt, seed=2)
     target = M.render(gt, res, size, cam, light_pos, P, np.float32)
     runs = []
-    for ps in (None, 0, res * res + 37):
+    for ps, own in ((None, False), (0, False), (res * res + 37, False), (None, True), (0, True)):
         fs, tex = make_problem(pkg, init, target, res, size, cam, light_pos, P, "log_l2", optim_light=light,
-                               plane_stride=ps)
+                               plane_stride=ps, own_textures=own)
         assert fs.plane_stride == (pkg._lib.query("svbrdf_plane_stride", res) if ps is None else ps)
+        assert fs.texture_stride == (fs.plane_stride if own else 0)
         for _ in range(4):
             fs.step()
         torch.cuda.synchronize()
-        runs.append((tex.cpu().numpy(), fs.exp_avg.cpu().numpy(), fs.exp_avg_sq.cpu().numpy(), fs.losses(),
-                     fs.light_pow.cpu().numpy()))
+        if own:
+            # the caller's tensor is untouched until write_back
+            assert np.array_equal(tex.cpu().numpy()[0], init)
+            fs.write_back()
+        runs.append((tex.cpu().numpy(), fs.textures.cpu().numpy(), fs.exp_avg.cpu().numpy(),
+                     fs.exp_avg_sq.cpu().numpy(), fs.losses(), fs.light_pow.cpu().numpy()))
     for other in runs[1:]:
         for a, b in zip(runs[0], other):
             np.testing.assert_array_equal(a, b)
