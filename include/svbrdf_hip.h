@@ -246,14 +246,15 @@ int svbrdf_images_to_u8(const float *images, int n, int height, int width, int c
 
 /* ------------------------------------------------------------------------
  * Batched step: many independent materials (uneven resolution and view count)
- * in one persistent launch.  The reference runs one material per process
+ * in one launch.  The reference runs one material per process
  * (src/scripts.py:67-97); this is the same per-pixel iteration applied to
  * every material of a batch at once.  Each material is one svbrdf_material
  * (device pointers, host-side sizes); the host plans tiles once, copies the
  * table and the two tile arrays to device memory, and then calls
- * svbrdf_batch_step once per iteration.  Blocks of a persistent grid take
- * tiles from an atomic ticket counter in cost order (views x pixels, largest
- * first); per-material losses are reduced in a fixed order (bit-reproducible).
+ * svbrdf_batch_step once per iteration.  Tiles run in cost order (views x
+ * pixels, largest first), either one block per 256-pixel tile or from an
+ * atomic ticket counter in a persistent grid; per-material losses are reduced
+ * in a fixed order (bit-reproducible).
  * Light-power optimisation is not available in batched mode.
  */
 typedef struct svbrdf_material {
@@ -270,8 +271,9 @@ typedef struct svbrdf_material {
     unsigned int tile_begin;        /* set by svbrdf_batch_plan                     */
     unsigned int n_tiles;           /* set by svbrdf_batch_plan                     */
     unsigned int plane_stride;      /* floats between the planes of exp_avg,
-                                       exp_avg_sq and targets_prepared; 0 = R*R
-                                       (textures are always dense)               */
+                                       exp_avg_sq and targets_prepared; 0 = R*R  */
+    unsigned int texture_stride;    /* floats between the planes of textures;
+                                       0 = R*R                                   */
 } svbrdf_material;
 
 /* Number of tiles of `tile_pixels` pixels (a multiple of 256) the batch splits
@@ -296,7 +298,10 @@ int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int vie
 /* One optimisation iteration of every material of the batch (device copies of
  * the table and tile arrays).  losses[m] receives material m's loss;
  * loss_history (optional) is [history_len][n_materials].  state is shared by
- * the batch (all materials step together). */
+ * the batch (all materials step together).  grid_blocks > 0 launches that
+ * many persistent blocks drawing tiles from the ticket counter; grid_blocks
+ * = 0 launches one block per tile instead (tile_pixels = 256, one pixel per
+ * thread), in cost order, and leaves the scheduling to the dispatcher. */
 int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const unsigned int *order,
                       const unsigned int *tile_material, size_t n_tiles, int tile_pixels,
                       int pixels_per_thread, int views_per_trip, int grid_blocks, int loss_mode,

@@ -1,17 +1,19 @@
-"""Many materials, one persistent launch per iteration.
+"""Many materials, one launch per iteration.
 
 The reference optimises one material per process (src/scripts.py:67-97).
 ``BatchedStep`` runs the same per-pixel iteration (clamp, render, loss,
 adjoint, Adam; src/svbrdf.py:58-71) for a whole batch of materials of
-uneven resolution and view count in one persistent kernel: blocks take
-fixed-size pixel tiles from an atomic ticket counter, costliest tiles
-(views x pixels) first.
+uneven resolution and view count in one kernel launch over a table of
+pixel tiles in cost order (views x pixels, costliest first).  Two launch
+forms: one block per 256-pixel tile (blocks_per_cu = -1, the hardware
+dispatcher balances the load), or a persistent grid whose blocks draw
+tiles of 1,024-16,384 pixels from an atomic ticket counter.
 
-The launch shape -- tile size, pixels per thread, views per loop trip and
-blocks per compute unit -- is chosen by :meth:`BatchedStep.autotune`, which
-times each candidate on the batch itself (state saved and restored around
-the trials) and caches the winner per (device, loss, batch shape) in a JSON
-file.
+The launch shape -- form, tile size, pixels per thread, views per loop trip
+and blocks per compute unit -- is chosen by :meth:`BatchedStep.autotune`,
+which times each candidate on the batch itself (state saved and restored
+around the trials) and caches the winner per (device, loss, batch shape) in
+a JSON file.
 """
 import ctypes
 import hashlib
@@ -25,7 +27,7 @@ import torch
 from . import _lib
 from .optim import _STATE_DTYPE, _planes, loss_mode
 
-DEFAULT_CONFIG = {"tile_pixels": 4096, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": 0}
+DEFAULT_CONFIG = {"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": -1}
 
 
 class Material(ctypes.Structure):
@@ -34,7 +36,7 @@ class Material(ctypes.Structure):
                 ("targets_prepared", ctypes.c_void_p), ("camera_pos", ctypes.c_void_p),
                 ("light_pos", ctypes.c_void_p), ("light_pow", ctypes.c_void_p), ("res", ctypes.c_int),
                 ("n_views", ctypes.c_int), ("size", ctypes.c_float), ("tile_begin", ctypes.c_uint),
-                ("n_tiles", ctypes.c_uint), ("plane_stride", ctypes.c_uint)]
+                ("n_tiles", ctypes.c_uint), ("plane_stride", ctypes.c_uint), ("texture_stride", ctypes.c_uint)]
 
 
 def _cache_path():
@@ -51,11 +53,14 @@ class BatchedStep:
 
     problems: sequence of (renderer, textures, targets) with renderer a
     Microfacet (geometry, light power, size), textures a contiguous fp32
-    CUDA tensor of 9*R*R values (updated in place) and targets [V,3,R,R].
-    All materials share lr, betas, eps and the loss, and step together.
+    CUDA tensor of 9*R*R values (updated in place, or with own_textures=True
+    copied into a buffer at the padded plane stride that write_back()
+    returns) and targets [V,3,R,R].  All materials share lr, betas, eps and
+    the loss, and step together.
     """
 
-    def __init__(self, problems, lr, loss="log_l2", betas=(0.9, 0.999), eps=1e-8, history=0, config=None):
+    def __init__(self, problems, lr, loss="log_l2", betas=(0.9, 0.999), eps=1e-8, history=0, config=None,
+                 own_textures=False):
         if not problems:
             raise ValueError("empty batch")
         self.device = problems[0][0].device
@@ -63,7 +68,7 @@ class BatchedStep:
         self.mode = loss_mode(loss)
         s = _lib.stream_handle(self.device)
         self.renderers, self.textures, self.exp_avg, self.exp_avg_sq, self.targets = [], [], [], [], []
-        self.plane_strides = []
+        self.plane_strides, self.sources = [], []
         for r, tex, tgt in problems:
             if tex.device != self.device or tex.dtype != torch.float32 or not tex.is_contiguous():
                 raise ValueError("textures must be contiguous float32 tensors on one CUDA device")
@@ -79,6 +84,12 @@ class BatchedStep:
                                tg.shape, ps)
             prepared.copy_(dense)
             self.renderers.append(r)
+            self.sources.append(tex)
+            if own_textures:
+                own = _planes(torch.zeros(9 * ps, dtype=torch.float32, device=self.device), tex.shape, ps)
+                with torch.no_grad():
+                    own.copy_(tex)
+                tex = own
             self.textures.append(tex)
             self.plane_strides.append(ps)
             self.exp_avg.append(_planes(moments[0], tex.shape, ps))
@@ -87,6 +98,7 @@ class BatchedStep:
         self.light_pow = [r.light_pow.detach().to(self.device, torch.float32).reshape(3).contiguous()
                           for r in self.renderers]
         self.n = len(problems)
+        self.own_textures = bool(own_textures)
         nstate = _lib.query("svbrdf_step_state_bytes")
         self.state = torch.zeros((nstate + 15) // 16 * 4, dtype=torch.float32, device=self.device)
         _lib.call("svbrdf_step_state_init", _lib.ptr(self.state), self.beta1, self.beta2, s)
@@ -94,6 +106,7 @@ class BatchedStep:
         self.history = torch.zeros((max(int(history), 0), self.n), dtype=torch.float32, device=self.device)
         self.steps_launched = 0
         self.graph = None
+        self.graph_steps = 0
         self.cu = _compute_units(self.device)
         self.configure(dict(DEFAULT_CONFIG, **(config or {})))
 
@@ -110,18 +123,26 @@ class BatchedStep:
         tile = int(cfg["tile_pixels"])
         if tile % (256 * npix_t):
             raise ValueError(f"tile_pixels {tile} is not a multiple of {256 * npix_t}")
-        max_bpc = _lib.query("svbrdf_batch_max_blocks_per_cu", self.mode, npix_t, unroll)
-        if max_bpc < 1:
-            raise RuntimeError("svbrdf_batch_max_blocks_per_cu failed: " + _lib.load().svbrdf_last_error().decode())
         bpc = int(cfg.get("blocks_per_cu") or 0)
-        bpc = max_bpc if bpc <= 0 else min(bpc, max_bpc)
-        cfg["blocks_per_cu"] = bpc
+        if bpc < 0:
+            # one block per 256-pixel tile, no persistent loop
+            if tile != 256 or npix_t != 1:
+                raise ValueError("blocks_per_cu=-1 (one block per tile) needs tile_pixels=256, pixels_per_thread=1")
+            cfg["blocks_per_cu"] = -1
+        else:
+            max_bpc = _lib.query("svbrdf_batch_max_blocks_per_cu", self.mode, npix_t, unroll)
+            if max_bpc < 1:
+                raise RuntimeError("svbrdf_batch_max_blocks_per_cu failed: "
+                                   + _lib.load().svbrdf_last_error().decode())
+            bpc = max_bpc if bpc == 0 else min(bpc, max_bpc)
+            cfg["blocks_per_cu"] = bpc
         table = (Material * self.n)()
         for i, r in enumerate(self.renderers):
             table[i] = Material(self.textures[i].data_ptr(), self.exp_avg[i].data_ptr(),
                                 self.exp_avg_sq[i].data_ptr(), self.targets[i].data_ptr(),
                                 r.camera_pos.data_ptr(), r.light_pos.data_ptr(), self.light_pow[i].data_ptr(),
-                                r.res, r.n_of_imgs, r.size, 0, 0, self.plane_strides[i])
+                                r.res, r.n_of_imgs, r.size, 0, 0, self.plane_strides[i],
+                                self.plane_strides[i] if self.own_textures else 0)
         n_tiles = _lib.query("svbrdf_batch_tile_count", table, self.n, tile)
         if n_tiles == 0:
             raise ValueError("invalid batch for tile planning")
@@ -139,7 +160,7 @@ class BatchedStep:
         self.ws_bytes = self.workspace.numel() * 4 - off
         self.n_tiles = n_tiles
         self.config = cfg
-        self.grid = bpc * self.cu
+        self.grid = 0 if bpc < 0 else bpc * self.cu
         self.graph = None
         return cfg
 
@@ -157,21 +178,35 @@ class BatchedStep:
         self._launch(_lib.stream_handle(self.device))
         self.steps_launched += 1
 
-    def capture(self):
+    def capture(self, steps=1):
+        """Records `steps` consecutive iterations in one HIP graph; every
+        replay() runs that many (a graph launch costs tens of microseconds)."""
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             g.capture_begin()
-            self._launch(ctypes.c_void_p(s.cuda_stream))
+            for _ in range(steps):
+                self._launch(ctypes.c_void_p(s.cuda_stream))
             g.capture_end()
         torch.cuda.current_stream().wait_stream(s)
         self.graph = g
+        self.graph_steps = steps
         return g
 
     def replay(self):
         self.graph.replay()
-        self.steps_launched += 1
+        self.steps_launched += self.graph_steps
+
+    def write_back(self):
+        """Copies each material's maps into the tensor it was given when the
+        batch keeps its own copies (own_textures=True); asynchronous."""
+        if self.own_textures:
+            with torch.no_grad():
+                for src, own in zip(self.sources, self.textures):
+                    src.copy_(own)
 
     def losses(self):
         """Per-material losses of the last step (synchronises)."""
@@ -207,6 +242,9 @@ class BatchedStep:
     @staticmethod
     def candidates():
         out = []
+        for unroll in (1, 3):
+            # one block per 256-pixel tile (blocks_per_cu = -1)
+            out.append({"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": unroll, "blocks_per_cu": -1})
         for npix_t in (1, 2):
             for unroll in (1, 3):
                 for tile in (1024, 4096, 16384):
@@ -215,9 +253,10 @@ class BatchedStep:
                                     "blocks_per_cu": bpc})
         return out
 
-    def autotune(self, candidates=None, iters=3, use_cache=True, verbose=False):
+    def autotune(self, candidates=None, iters=5, use_cache=True, verbose=False, rounds=2):
         """Times each candidate launch shape on this batch and keeps the
-        fastest; the optimisation state is restored afterwards.  Returns
+        fastest (best of `rounds` passes over the candidates, `iters` steps
+        each); the optimisation state is restored afterwards.  Returns
         (config, {config key: ms per step})."""
         key = self.signature()
         path = _cache_path()
@@ -229,29 +268,31 @@ class BatchedStep:
             if cached:
                 return self.configure(cached), {}
         snap = self._snapshot()
-        timings = {}
-        best = None
+        shapes = {}
         for cand in candidates or self.candidates():
             cfg = self.configure(cand)
-            k = json.dumps(cfg, sort_keys=True)
-            if k in timings:
-                continue
-            for _ in range(2):
-                self.step()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize(self.device)
-            e0.record()
-            for _ in range(iters):
-                self.step()
-            e1.record()
-            torch.cuda.synchronize(self.device)
-            ms = e0.elapsed_time(e1) / iters
-            timings[k] = ms
-            if verbose:
-                print(f"autotune {k}: {ms:.3f} ms", flush=True)
-            if best is None or ms < best[0]:
-                best = (ms, cfg)
-            self._restore(snap)
+            shapes.setdefault(json.dumps(cfg, sort_keys=True), cfg)
+        timings = {}
+        for _ in range(max(1, rounds)):
+            for k, cfg in shapes.items():
+                # timed as the batch runs: replays of a graph of `iters` steps
+                self.configure(cfg)
+                g = self.capture(iters)
+                g.replay()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize(self.device)
+                e0.record()
+                g.replay()
+                e1.record()
+                torch.cuda.synchronize(self.device)
+                ms = e0.elapsed_time(e1) / iters
+                del g
+                self.graph = None
+                timings[k] = min(ms, timings.get(k, float("inf")))
+                if verbose:
+                    print(f"autotune {k}: {ms:.3f} ms", flush=True)
+                self._restore(snap)
+        best = (timings[min(timings, key=timings.get)], shapes[min(timings, key=timings.get)])
         self._restore(snap)
         cfg = self.configure(best[1])
         if use_cache:

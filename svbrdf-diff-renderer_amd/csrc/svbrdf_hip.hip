@@ -895,6 +895,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
         const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
         const unsigned first = (id - mt.tile_begin) * (unsigned)tile_pixels;
         const unsigned ps = mt.plane_stride ? mt.plane_stride : npix;
+        const unsigned tps = mt.texture_stride ? mt.texture_stride : npix;
         const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
         const float P[3] = {ldu(mt.light_pow, 0), ldu(mt.light_pow, 1), ldu(mt.light_pow, 2)};
         const bool coloc = all_colocated(mt.camera_pos, mt.light_pos, mt.n_views);
@@ -907,11 +908,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
                 px[q] = pixel_at(first + base + q * kBlock + threadIdx.x, npix, mt.res, mt.size);
             if (coloc)
                 fused_pixels<LOSS, false, true, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
-                                                              mt.targets_prepared, npix, ps, mt.n_views, mt.camera_pos,
+                                                              mt.targets_prepared, tps, ps, mt.n_views, mt.camera_pos,
                                                               mt.light_pos, P, K2, sc, loss, gPs);
             else
                 fused_pixels<LOSS, false, false, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
-                                                               mt.targets_prepared, npix, ps, mt.n_views, mt.camera_pos,
+                                                               mt.targets_prepared, tps, ps, mt.n_views, mt.camera_pos,
                                                                mt.light_pos, P, K2, sc, loss, gPs);
         }
         // block_sum ends with a barrier, so every thread has read s_slot before
@@ -924,6 +925,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
         __syncthreads();
         slot = __builtin_amdgcn_readfirstlane(s_slot);
     }
+}
+
+// One block per tile instead of a persistent grid: block b takes the tile in
+// cost order b (the dispatcher hands blocks out in index order, so the
+// costliest tiles start first), with one pixel per thread (256-pixel tiles).
+// No ticket counter and no loop; the register budget is the single-material
+// kernel's.
+template <int LOSS, int UNROLL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_FUSED_WAVES))) void batch_grid_kernel(
+    const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
+    const unsigned *__restrict__ tile_mat, float4 *__restrict__ partials, double lr, double beta1, double beta2,
+    double eps, const svbrdf_step_state *__restrict__ st) {
+    __shared__ float lds[kBlock / 64];
+    const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+    const unsigned id = order[blockIdx.x];
+    const svbrdf_material mt = mats[tile_mat[id]];
+    const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
+    const unsigned ps = mt.plane_stride ? mt.plane_stride : npix;
+    const unsigned tps = mt.texture_stride ? mt.texture_stride : npix;
+    const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
+    const float P[3] = {ldu(mt.light_pow, 0), ldu(mt.light_pow, 1), ldu(mt.light_pow, 2)};
+    float loss = 0.f;
+    float gPs[3] = {0.f, 0.f, 0.f};
+    Pix px[1];
+    px[0] = pixel_at((id - mt.tile_begin) * (unsigned)kBlock + threadIdx.x, npix, mt.res, mt.size);
+    if (all_colocated(mt.camera_pos, mt.light_pos, mt.n_views))
+        fused_pixels<LOSS, false, true, 1, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq, mt.targets_prepared,
+                                                   tps, ps, mt.n_views, mt.camera_pos, mt.light_pos, P, K2, sc, loss,
+                                                   gPs);
+    else
+        fused_pixels<LOSS, false, false, 1, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq, mt.targets_prepared,
+                                                    tps, ps, mt.n_views, mt.camera_pos, mt.light_pos, P, K2, sc, loss,
+                                                    gPs);
+    const float bl = block_sum(loss, lds);
+    if (threadIdx.x == 0) partials[id] = make_float4(bl, 0.f, 0.f, 0.f);
 }
 
 // Block m sums material m's tile partials in tile order (fixed, so the loss is
@@ -1196,6 +1232,7 @@ int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixe
             return fail(SVBRDF_ESHAPE, "svbrdf_batch_plan: material %d has res=%d n_views=%d", i, m.res, m.n_views);
         unsigned ps = 0;
         if (int rc = check_plane_stride("svbrdf_batch_plan", m.res, m.plane_stride, ps)) return rc;
+        if (int rc = check_plane_stride("svbrdf_batch_plan", m.res, m.texture_stride, ps)) return rc;
     }
     const size_t need = svbrdf_batch_tile_count(materials, n_materials, tile_pixels);
     if (need != n_tiles)
@@ -1248,7 +1285,10 @@ int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const u
     if (tile_pixels < kBlock * pixels_per_thread || tile_pixels % (kBlock * pixels_per_thread))
         return fail(SVBRDF_EINVAL, "%s: tile_pixels=%d must be a multiple of %d", what, tile_pixels,
                     kBlock * pixels_per_thread);
-    if (grid_blocks < 1 || grid_blocks > (1 << 20)) return fail(SVBRDF_EINVAL, "%s: grid_blocks=%d", what, grid_blocks);
+    if (grid_blocks < 0 || grid_blocks > (1 << 20)) return fail(SVBRDF_EINVAL, "%s: grid_blocks=%d", what, grid_blocks);
+    if (grid_blocks == 0 && (tile_pixels != kBlock || pixels_per_thread != 1 || n_tiles > 0x7fffffffull))
+        return fail(SVBRDF_EINVAL, "%s: one block per tile (grid_blocks=0) needs tile_pixels=%d, one pixel per "
+                    "thread and fewer than 2^31 tiles", what, kBlock);
     if (!(lr >= 0.0) || !(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0) || !(eps >= 0.0))
         return fail(SVBRDF_EINVAL, "%s: invalid Adam hyper-parameters", what);
     if (workspace_bytes < svbrdf_batch_workspace_bytes(n_tiles))
@@ -1271,8 +1311,23 @@ int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const u
             if (views_per_trip == 3) SVBRDF_LAUNCH_BATCH(L, 2, 3); else SVBRDF_LAUNCH_BATCH(L, 2, 1);          \
         }                                                                                                      \
     } while (0)
-    if (loss_mode == SVBRDF_LOSS_LOG_L2) SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_LOG_L2);
-    else SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_L2);
+#define SVBRDF_LAUNCH_GRID(L, U)                                                                              \
+    hipLaunchKernelGGL((batch_grid_kernel<L, U>), dim3((unsigned)n_tiles), dim3(kBlock), 0, s, materials, order, \
+                       tile_material, partials, lr, beta1, beta2, eps, st)
+    if (grid_blocks == 0) {
+        if (loss_mode == SVBRDF_LOSS_LOG_L2) {
+            if (views_per_trip == 3) SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_LOG_L2, 3);
+            else SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_LOG_L2, 1);
+        } else {
+            if (views_per_trip == 3) SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_L2, 3);
+            else SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_L2, 1);
+        }
+    } else if (loss_mode == SVBRDF_LOSS_LOG_L2) {
+        SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_LOG_L2);
+    } else {
+        SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_L2);
+    }
+#undef SVBRDF_LAUNCH_GRID
 #undef SVBRDF_LAUNCH_BATCH_L
 #undef SVBRDF_LAUNCH_BATCH
     if ((rc = check_launch(what))) return rc;

@@ -54,6 +54,8 @@ def build(pkg, specs, kind, **kw):
     {"tile_pixels": 4096, "pixels_per_thread": 1, "views_per_trip": 3, "blocks_per_cu": 0},
     {"tile_pixels": 1024, "pixels_per_thread": 2, "views_per_trip": 1, "blocks_per_cu": 2},
     {"tile_pixels": 16384, "pixels_per_thread": 2, "views_per_trip": 3, "blocks_per_cu": 0},
+    {"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": -1},
+    {"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": 3, "blocks_per_cu": -1},
 ])
 @pytest.mark.parametrize("loss", ["log_l2", "l2"])
 def test_batch_matches_single_material_steps(pkg, cfg, loss):
@@ -101,6 +103,30 @@ def test_batch_graph_replay_and_determinism(pkg):
     np.testing.assert_array_equal(a.loss_history(), b.loss_history())
 
 
+def test_batch_own_textures_and_multi_step_graphs(pkg):
+    """Maps at the padded plane stride (own_textures) and graphs of several
+    iterations: the same arithmetic as one eager launch per step on the
+    caller's tensors."""
+    specs = make_problems(pkg, SHAPES, seed=9)
+    a, ta = build(pkg, specs, "batch", history=8)
+    b, tb = build(pkg, specs, "batch", history=8, own_textures=True)
+    for _ in range(6):
+        a.step()
+    b.capture(3)
+    b.replay()
+    b.replay()
+    torch.cuda.synchronize()
+    for t, spec in zip(tb, specs):
+        assert np.array_equal(t[0].cpu().numpy(), spec[6])      # untouched until write_back
+    b.write_back()
+    torch.cuda.synchronize()
+    for i, (x, y) in enumerate(zip(ta, tb)):
+        np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy())
+        np.testing.assert_array_equal(a.exp_avg_sq[i].cpu().numpy(), b.exp_avg_sq[i].cpu().numpy())
+    np.testing.assert_array_equal(a.loss_history(), b.loss_history())
+    assert b.steps_launched == 6 and b.read_state()["step"] == 6.0
+
+
 def test_autotune_restores_state_and_picks_a_candidate(pkg, tmp_path, monkeypatch):
     monkeypatch.setenv("SVBRDF_AMD_CACHE", str(tmp_path))
     specs = make_problems(pkg, SHAPES[:4], seed=3)
@@ -112,7 +138,7 @@ def test_autotune_restores_state_and_picks_a_candidate(pkg, tmp_path, monkeypatc
     cands = pkg.BatchedStep.candidates()[:6]
     cfg, timings = b.autotune(candidates=cands, iters=2)
     assert len(timings) >= 1 and all(t > 0 for t in timings.values())
-    assert cfg["tile_pixels"] in (1024, 4096, 16384)
+    assert cfg in [dict(c, blocks_per_cu=cfg["blocks_per_cu"]) for c in cands]
     after = [t.cpu().numpy() for t in tex]
     for x, y in zip(before, after):
         np.testing.assert_array_equal(x, y)

@@ -73,6 +73,35 @@ def test_batch_plan_orders_tiles_by_cost(pkg):
     assert costs == sorted(costs, reverse=True)
     assert lib.svbrdf_batch_plan(mats, len(shapes), 4096, (ctypes.c_uint * 3)(), tile_mat, 3) == -3
     assert lib.svbrdf_batch_plan(mats, len(shapes), 1000, (ctypes.c_uint * n)(), tile_mat, n) == -1
+    # plane strides below res^2 are refused, padded ones accepted
+    mats[0].plane_stride = 256 * 256 - 1
+    assert lib.svbrdf_batch_plan(mats, len(shapes), 4096, order_arr(n), tile_mat, n) == -1
+    mats[0].plane_stride = lib.svbrdf_plane_stride(256)
+    mats[0].texture_stride = 100
+    assert lib.svbrdf_batch_plan(mats, len(shapes), 4096, order_arr(n), tile_mat, n) == -1
+    mats[0].texture_stride = lib.svbrdf_plane_stride(256)
+    assert lib.svbrdf_batch_plan(mats, len(shapes), 4096, order_arr(n), tile_mat, n) == 0
+
+
+def order_arr(n):
+    return (ctypes.c_uint * n)()
+
+
+def test_batch_step_one_block_per_tile_needs_256_pixel_tiles(pkg):
+    """grid_blocks = 0 (one block per tile) is refused on the host for other
+    tile sizes or two pixels per thread, before anything is launched."""
+    lib = pkg._lib.load()
+    fake = ctypes.c_void_p(0x10000)
+    ws = lib.svbrdf_batch_workspace_bytes(64)
+
+    def step(tile, npix_t, grid):
+        return lib.svbrdf_batch_step(fake, 1, fake, fake, 64, tile, npix_t, 1, grid, 1, ctypes.c_double(0.01),
+                                     ctypes.c_double(0.9), ctypes.c_double(0.999), ctypes.c_double(1e-8),
+                                     fake, fake, None, 0, fake, ws, None)
+    assert step(1024, 1, 0) == -1
+    assert step(512, 2, 0) == -1
+    assert step(256, 1, -1) == -1
+    assert b"one block per tile" in lib.svbrdf_last_error() or b"grid_blocks" in lib.svbrdf_last_error()
 
 
 def test_bench_eager_baseline_renders_the_reference_images():
