@@ -145,6 +145,16 @@ __device__ __forceinline__ Pix pixel_at(unsigned p, unsigned npix, int res, floa
     x.valid = p < npix;
     x.p = x.valid ? p : npix - 1u;
     x.off = x.p * 4u;
+    if ((res & (res - 1)) == 0) {
+        // power-of-two R (a wave-uniform branch): the row is a shift and the
+        // division by R an exponent change, both exact, so the coordinates
+        // are the IEEE-division ones bit for bit
+        const int k = __builtin_ctz((unsigned)res);
+        const unsigned i = x.p >> k, j = x.p & (unsigned)(res - 1);
+        x.px = (ldexpf((float)j + 0.5f, -k) - 0.5f) * size;
+        x.py = -((ldexpf((float)i + 0.5f, -k) - 0.5f) * size);
+        return x;
+    }
     const unsigned i = x.p / (unsigned)res, j = x.p - i * (unsigned)res;
     x.px = pixel_coord((int)j, res, size);
     x.py = -pixel_coord((int)i, res, size);
