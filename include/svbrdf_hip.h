@@ -274,6 +274,7 @@ typedef struct svbrdf_material {
                                        exp_avg_sq and targets_prepared; 0 = R*R  */
     unsigned int texture_stride;    /* floats between the planes of textures;
                                        0 = R*R                                   */
+    float loss_scale;               /* set by svbrdf_batch_plan                   */
 } svbrdf_material;
 
 /* Number of tiles of `tile_pixels` pixels (a multiple of 256) the batch splits

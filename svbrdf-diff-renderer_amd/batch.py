@@ -36,7 +36,8 @@ class Material(ctypes.Structure):
                 ("targets_prepared", ctypes.c_void_p), ("camera_pos", ctypes.c_void_p),
                 ("light_pos", ctypes.c_void_p), ("light_pow", ctypes.c_void_p), ("res", ctypes.c_int),
                 ("n_views", ctypes.c_int), ("size", ctypes.c_float), ("tile_begin", ctypes.c_uint),
-                ("n_tiles", ctypes.c_uint), ("plane_stride", ctypes.c_uint), ("texture_stride", ctypes.c_uint)]
+                ("n_tiles", ctypes.c_uint), ("plane_stride", ctypes.c_uint), ("texture_stride", ctypes.c_uint),
+                ("loss_scale", ctypes.c_float)]
 
 
 def _cache_path():

@@ -412,15 +412,41 @@ struct StepScalars {
     float w1, beta2, om_beta2, inv_bc2_sqrt, eps, neg_step;
 };
 
-__device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st, double lr, double beta1,
-                                                    double beta2, double eps) {
+// Launch constants of a step, computed once on the host in double (as
+// torch's Python floats are) and rounded once to fp32.  In the kernel every
+// wave would redo these double divisions and conversions: on the VALU, at
+// the double rate.
+struct StepConsts {
+    double lr, beta1, beta2, eps;   // as given (state advance, light-power Adam)
+    double count;                   // image-loss elements, V * 3 * R^2
+    float K2;                       // d mean / d element times 1/2.2: (2 / count) / 2.2
+    float w1, beta2_f, om_beta2, eps_f;
+};
+
+StepConsts make_consts(double lr, double beta1, double beta2, double eps, double count) {
+    StepConsts c;
+    c.lr = lr; c.beta1 = beta1; c.beta2 = beta2; c.eps = eps; c.count = count;
+    c.K2 = count > 0.0 ? (float)(2.0 / count / 2.2) : 0.0f;
+    c.w1 = (float)(1.0 - beta1);
+    c.beta2_f = (float)beta2;
+    c.om_beta2 = (float)(1.0 - beta2);
+    c.eps_f = (float)eps;
+    return c;
+}
+
+// K2 of one material of a batch (host side, stored in its table entry).
+float loss_scale(int res, int n_views) {
+    return (float)(2.0 / ((double)n_views * 3.0 * (double)res * (double)res) / 2.2);
+}
+
+__device__ __forceinline__ StepScalars step_scalars(const svbrdf_step_state *st, const StepConsts &c) {
     StepScalars s;
-    s.w1 = (float)(1.0 - beta1);
-    s.beta2 = (float)beta2;
-    s.om_beta2 = (float)(1.0 - beta2);
+    s.w1 = c.w1;
+    s.beta2 = c.beta2_f;
+    s.om_beta2 = c.om_beta2;
     s.inv_bc2_sqrt = st->inv_bias_correction2_sqrt;
-    s.eps = (float)eps;
-    s.neg_step = (float)(-lr * st->inv_bias_correction1);
+    s.eps = c.eps_f;
+    s.neg_step = (float)(-c.lr * st->inv_bias_correction1);
     return s;
 }
 
@@ -773,15 +799,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     float *__restrict__ tex, float *__restrict__ exp_avg, float *__restrict__ exp_avg_sq,
     const float *__restrict__ tgt, int res, unsigned tps, unsigned ps, int n_views, float size,
     const float *__restrict__ cam,
-    const float *__restrict__ light, float *__restrict__ lpow, double lr, double beta1, double beta2,
-    double eps, svbrdf_step_state *st, void *ws, float *loss_history, int history_len) {
+    const float *__restrict__ light, float *__restrict__ lpow, StepConsts c, svbrdf_step_state *st, void *ws,
+    float *loss_history, int history_len) {
     constexpr int NPIX = SVBRDF_FUSED_NPIX;
     __shared__ float lds[kBlock / 64];
     const unsigned npix = (unsigned)res * (unsigned)res;
-    const double count = (double)n_views * 3.0 * (double)npix;
+    const double count = c.count;
     // d mean / d element, times d out^... / d ln img = 1/2.2 (folded once)
-    const float K2 = (float)(2.0 / count / 2.2);
-    const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+    const float K2 = c.K2;
+    const StepScalars sc = step_scalars(st, c);
     const float step_f = st->step;
     const unsigned t = (unsigned)step_f;
     if (blockIdx.x == 0 && t >= 1 && st->pending_step == t) {
@@ -822,7 +848,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
         if (atomicAdd(counters, 1u) != n_groups - 1) return;
         // every block of this launch has read the state
         __hip_atomic_store(counters, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        advance_step(st, step_f + 1.0f, beta1, beta2);
+        advance_step(st, step_f + 1.0f, c.beta1, c.beta2);
         st->pending_step = t + 1;
     }
 }
@@ -832,9 +858,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
 // light-power Adam step (one param group with the maps, src/svbrdf.py:50)
 // and advances the state.
 template <bool OPT_LIGHT>
-__global__ __launch_bounds__(kBlock) void finalize_step_kernel(void *ws, unsigned n_partials, double count,
-                                                              float *__restrict__ lpow, double lr, double beta1,
-                                                              double beta2, double eps, svbrdf_step_state *st,
+__global__ __launch_bounds__(kBlock) void finalize_step_kernel(void *ws, unsigned n_partials, StepConsts k,
+                                                              float *__restrict__ lpow, svbrdf_step_state *st,
                                                               float *loss_history, int history_len) {
     constexpr int NQ = OPT_LIGHT ? 4 : 1;
     __shared__ double lds[NQ][kBlock / 64];
@@ -843,10 +868,10 @@ __global__ __launch_bounds__(kBlock) void finalize_step_kernel(void *ws, unsigne
     double tot[NQ];
     reduce_partials<NQ>(partial_buffer(ws, n_partials, t & 1u), n_partials, tot, lds);
     if (threadIdx.x != 0) return;
-    record_loss(st, t + 1, tot[0], count, loss_history, history_len);
+    record_loss(st, t + 1, tot[0], k.count, loss_history, history_len);
     if constexpr (OPT_LIGHT) {
         // Adam on light_pow with the same scalars as the maps
-        const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+        const StepScalars sc = step_scalars(st, k);
         for (int c = 0; c < 3; ++c) {
             float pp = lpow[c], mm = st->light_exp_avg[c], vv = st->light_exp_avg_sq[c];
             const float g = (float)tot[1 + c];
@@ -857,7 +882,7 @@ __global__ __launch_bounds__(kBlock) void finalize_step_kernel(void *ws, unsigne
             st->light_grad[c] = g;
         }
     }
-    advance_step(st, step_f + 1.0f, beta1, beta2);
+    advance_step(st, step_f + 1.0f, k.beta1, k.beta2);
     st->pending_step = 0u;
 }
 
@@ -887,11 +912,10 @@ template <int LOSS, int NPIX, int UNROLL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 1 ? SVBRDF_BATCH_WAVES : 1))) void batch_step_kernel(
     const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
     const unsigned *__restrict__ tile_mat, unsigned n_tiles, int tile_pixels, unsigned *__restrict__ queue,
-    float4 *__restrict__ partials, double lr, double beta1, double beta2, double eps,
-    const svbrdf_step_state *__restrict__ st) {
+    float4 *__restrict__ partials, StepConsts c, const svbrdf_step_state *__restrict__ st) {
     __shared__ float lds[kBlock / 64];
     __shared__ unsigned s_slot;
-    const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+    const StepScalars sc = step_scalars(st, c);
     // The ticket is read back through readfirstlane so that the loop exit is a
     // scalar (wave-uniform) branch: with a divergent exit the control-flow
     // structurizer may keep lanes re-running a tile while lane 0 waits to draw
@@ -906,7 +930,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
         const unsigned first = (id - mt.tile_begin) * (unsigned)tile_pixels;
         const unsigned ps = mt.plane_stride ? mt.plane_stride : npix;
         const unsigned tps = mt.texture_stride ? mt.texture_stride : npix;
-        const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
+        const float K2 = mt.loss_scale;
         const float P[3] = {ldu(mt.light_pow, 0), ldu(mt.light_pow, 1), ldu(mt.light_pow, 2)};
         const bool coloc = all_colocated(mt.camera_pos, mt.light_pos, mt.n_views);
         float loss = 0.f;
@@ -945,16 +969,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
 template <int LOSS, int UNROLL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_FUSED_WAVES))) void batch_grid_kernel(
     const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
-    const unsigned *__restrict__ tile_mat, float4 *__restrict__ partials, double lr, double beta1, double beta2,
-    double eps, const svbrdf_step_state *__restrict__ st) {
+    const unsigned *__restrict__ tile_mat, float4 *__restrict__ partials, StepConsts c,
+    const svbrdf_step_state *__restrict__ st) {
     __shared__ float lds[kBlock / 64];
-    const StepScalars sc = step_scalars(st, lr, beta1, beta2, eps);
+    const StepScalars sc = step_scalars(st, c);
     const unsigned id = order[blockIdx.x];
     const svbrdf_material mt = mats[tile_mat[id]];
     const unsigned npix = (unsigned)mt.res * (unsigned)mt.res;
     const unsigned ps = mt.plane_stride ? mt.plane_stride : npix;
     const unsigned tps = mt.texture_stride ? mt.texture_stride : npix;
-    const float K2 = (float)(2.0 / ((double)mt.n_views * 3.0 * (double)npix) / 2.2);
+    const float K2 = mt.loss_scale;
     const float P[3] = {ldu(mt.light_pow, 0), ldu(mt.light_pow, 1), ldu(mt.light_pow, 2)};
     float loss = 0.f;
     float gPs[3] = {0.f, 0.f, 0.f};
@@ -1174,10 +1198,11 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
     const unsigned nb = n_blocks(res, SVBRDF_FUSED_NPIX);
     hipStream_t s = (hipStream_t)stream;
     svbrdf_step_state *st = (svbrdf_step_state *)state;
+    const StepConsts consts = make_consts(lr, beta1, beta2, eps, (double)n_views * 3.0 * (double)res * (double)res);
 #define SVBRDF_LAUNCH_STEP(L, O)                                                                             \
     hipLaunchKernelGGL((fused_step_kernel<L, O>), dim3(nb), dim3(kBlock), 0, s, textures, exp_avg, exp_avg_sq, \
-                       targets_prepared, res, tps, ps, n_views, size, camera_pos, light_pos, light_pow, lr,    \
-                       beta1, beta2, eps, st, workspace, loss_history, history_len)
+                       targets_prepared, res, tps, ps, n_views, size, camera_pos, light_pos, light_pow, consts, \
+                       st, workspace, loss_history, history_len)
     if (loss_mode == SVBRDF_LOSS_LOG_L2) {
         if (optim_light) SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, true);
         else SVBRDF_LAUNCH_STEP(SVBRDF_LOSS_LOG_L2, false);
@@ -1188,9 +1213,8 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
 #undef SVBRDF_LAUNCH_STEP
     if ((rc = check_launch("svbrdf_fused_step"))) return rc;
     if (optim_light) {
-        const double count = (double)n_views * 3.0 * (double)res * (double)res;
-        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kBlock), 0, s, workspace, nb, count, light_pow,
-                           lr, beta1, beta2, eps, st, loss_history, history_len);
+        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kBlock), 0, s, workspace, nb, consts, light_pow,
+                           st, loss_history, history_len);
     }
     return check_launch("svbrdf_fused_step");
 }
@@ -1242,6 +1266,7 @@ int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixe
             return fail(SVBRDF_ESHAPE, "svbrdf_batch_plan: material %d has res=%d n_views=%d", i, m.res, m.n_views);
         unsigned ps = 0;
         if (int rc = check_plane_stride("svbrdf_batch_plan", m.res, m.plane_stride, ps)) return rc;
+        materials[i].loss_scale = loss_scale(m.res, m.n_views);
         if (int rc = check_plane_stride("svbrdf_batch_plan", m.res, m.texture_stride, ps)) return rc;
     }
     const size_t need = svbrdf_batch_tile_count(materials, n_materials, tile_pixels);
@@ -1308,11 +1333,12 @@ int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const u
         return fail(SVBRDF_EINVAL, "%s: workspace must be 256-byte and state 16-byte aligned", what);
     hipStream_t s = (hipStream_t)stream;
     float4 *partials = (float4 *)workspace;
+    const StepConsts consts = make_consts(lr, beta1, beta2, eps, 0.0);   // K2 is per material
     unsigned *queue = (unsigned *)((char *)workspace + batch_partials_bytes(n_tiles));
     svbrdf_step_state *st = (svbrdf_step_state *)state;
 #define SVBRDF_LAUNCH_BATCH(L, N, U)                                                                          \
     hipLaunchKernelGGL((batch_step_kernel<L, N, U>), dim3(grid_blocks), dim3(kBlock), 0, s, materials, order, \
-                       tile_material, (unsigned)n_tiles, tile_pixels, queue, partials, lr, beta1, beta2, eps, st)
+                       tile_material, (unsigned)n_tiles, tile_pixels, queue, partials, consts, st)
 #define SVBRDF_LAUNCH_BATCH_L(L)                                                                              \
     do {                                                                                                       \
         if (pixels_per_thread == 1) {                                                                          \
@@ -1323,7 +1349,7 @@ int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const u
     } while (0)
 #define SVBRDF_LAUNCH_GRID(L, U)                                                                              \
     hipLaunchKernelGGL((batch_grid_kernel<L, U>), dim3((unsigned)n_tiles), dim3(kBlock), 0, s, materials, order, \
-                       tile_material, partials, lr, beta1, beta2, eps, st)
+                       tile_material, partials, consts, st)
     if (grid_blocks == 0) {
         if (loss_mode == SVBRDF_LOSS_LOG_L2) {
             if (views_per_trip == 3) SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_LOG_L2, 3);
