@@ -27,6 +27,7 @@ Printed JSON line (one):
   cpu_baseline      the same eager restatement on the host CPU cores (oracle/);
                     cpu_baseline_256 the same at 256^2 (config 0)
   forward           forward-render throughput at the same shape (config 1)
+  adjoint           the render adjoint alone (dL/dimages -> dL/dtextures, 180 B/pixel)
 
   batched           config 3: 64 materials of uneven size in one persistent
                     launch (autotuned), against a per-material loop
@@ -76,6 +77,11 @@ def algorithmic_bytes_step(res, views):
 
 def algorithmic_bytes_render(res, views):
     return res * res * (36 + 12 * views)
+
+
+def algorithmic_bytes_adjoint(res, views):
+    # textures and dL/dimages read once, dL/dtextures written once
+    return res * res * (36 + 12 * views + 36)
 
 
 def make_problem(pkg, res, views, dev, loss="log_l2"):
@@ -557,6 +563,24 @@ def main():
     out["forward"] = {"value": round(1000.0 / fms, 1), "unit": "renders/s", "ms": round(fms, 5),
                       "achieved_GBs": round(fb / (fms * 1e-3) / 1e9, 1),
                       "frac": round(fb / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    # the render adjoint on its own (RenderFunction.backward: dL/dimages ->
+    # dL/dtextures), as a torch-composed loss drives it
+    gout = torch.randn((views, 3, res, res), generator=torch.Generator().manual_seed(5)).to(dev)
+    gtex = torch.empty_like(gt)
+    L = pkg._lib
+    stream = L.stream_handle(dev)
+
+    def adjoint():
+        L.call("svbrdf_render_backward_terms", L.ptr(gt), res, views, r.size, L.ptr(r.camera_pos),
+               L.ptr(r.light_pos), L.ptr(r.light_pow), 0, L.ptr(gout), L.ptr(gtex), None, None, 0, stream)
+    for _ in range(5):
+        adjoint()
+    bms = time_events(adjoint, 50)
+    bb = algorithmic_bytes_adjoint(res, views)
+    out["adjoint"] = {"value": round(1000.0 / bms, 1), "unit": "adjoints/s", "ms": round(bms, 5),
+                      "achieved_GBs": round(bb / (bms * 1e-3) / 1e9, 1),
+                      "frac": round(bb / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     if not args.no_batch:
         out["batched"] = batch_leg(pkg, dev)

@@ -278,6 +278,18 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
     for (int c = 0; c < 9; ++c) t[c] = ld(tensor(tex, 9, npix), c, px.off);
     Maps m;
     decode(t, m);
+    // co-located: img_c = (P_c f1_c + DGq P_c F_c) (n.l)/d^2, the light power
+    // folded into the view-independent factors.  The loop accumulates
+    // dL/d(P_c f_c) without the constant 1/2.2 of d out/d img; P_c and 1/2.2
+    // are applied once after the loop (every accumulator is linear in them).
+    float Pf1[3], PF[3];
+    if constexpr (COLOC) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Pf1[c] = P[c] * m.f1[c];
+            PF[c] = P[c] * m.Fc[c];
+        }
+    }
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
         const Tensor G = tensor(gout + (size_t)(3 * k) * npix, 3, npix);
@@ -296,13 +308,17 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
             float gf[3], gnl = 0.f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float f = m.f1[c] + s.DGq * m.Fc[c];
-                const float gimg = dimg(P[c] * f * s.nl_d2, go[c]);
-                gf[c] = gimg * P[c] * s.nl_d2;
-                gnl += gimg * P[c] * f;
-                if (LIGHT_GRAD) a.gP[c] += gimg * f * s.nl_d2;
+                const float pf = Pf1[c] + s.DGq * PF[c];
+                const float img = pf * s.nl_d2;
+                const float ic = fminf(fmaxf(img, kEps), 1.0f);
+                // 2.2 d out/d img = ic^(1/2.2 - 1) inside the clamp, 0 outside
+                // (a NaN img is outside, as torch's clamp backward masks it)
+                const float gimg = ic == img ? go[c] * fexp2(flog2(ic) * (kInvGamma - 1.0f)) : 0.0f;
+                gf[c] = gimg * s.nl_d2;
+                gnl += gimg * pf;
+                if (LIGHT_GRAD) a.gP[c] += gimg * (m.f1[c] + s.DGq * m.Fc[c]) * s.nl_d2;
             }
-            backward_coloc(m, s, gf, gnl * s.inv_d2, frcp(fmaxf(s.nv, 1e-30f)), m.Fc, a);
+            backward_coloc(m, s, gf, gnl * s.inv_d2, frcp(fmaxf(s.nv, 1e-30f)), PF, a);
         } else {
             Geom g;
             geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
@@ -314,7 +330,17 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
             shade_backward<LIGHT_GRAD, TERMS>(m, g, s, P, gimg, a);
         }
     }
-    if constexpr (COLOC) coloc_finish(a);
+    if constexpr (COLOC) {
+        coloc_finish(a);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.S[c] *= kInvGamma * P[c];
+            a.T[c] *= kInvGamma * P[c];
+            a.gn[c] *= kInvGamma;
+            if (LIGHT_GRAD) a.gP[c] *= kInvGamma;
+        }
+        a.A *= kInvGamma;
+    }
     float gt[9];
     decode_backward(t, m, a, gt);
     if (px.valid) {
@@ -323,8 +349,11 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
     }
 }
 
+#ifndef SVBRDF_BACKWARD_WAVES
+#define SVBRDF_BACKWARD_WAVES 5      // minimum waves per SIMD for the adjoint (95 VGPRs, no spills)
+#endif
 template <bool LIGHT_GRAD, int TERMS = 0>
-__global__ __launch_bounds__(kBlock) void backward_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_BACKWARD_WAVES))) void backward_kernel(
     const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, const float *__restrict__ lpow, const float *__restrict__ gout,
     float *__restrict__ gtex, float *__restrict__ light_partials) {
@@ -1064,9 +1093,12 @@ int svbrdf_render_terms(const float *textures, int res, int n_views, float size,
     if (terms < 0 || terms > (SVBRDF_NDF_BECKMANN | SVBRDF_FRESNEL_SCHLICK))
         return fail(SVBRDF_EINVAL, "svbrdf_render: unknown terms %d", terms);
     hipStream_t s = (hipStream_t)stream;
+#ifndef SVBRDF_RENDER_NPIX
+#define SVBRDF_RENDER_NPIX 1
+#endif
 #define SVBRDF_LAUNCH_RENDER(T)                                                                               \
-    hipLaunchKernelGGL((render_kernel<1, T>), dim3(n_blocks(res)), dim3(kBlock), 0, s, textures, res, n_views, \
-                       size, camera_pos, light_pos, light_pow, images)
+    hipLaunchKernelGGL((render_kernel<SVBRDF_RENDER_NPIX, T>), dim3(n_blocks(res, SVBRDF_RENDER_NPIX)),         \
+                       dim3(kBlock), 0, s, textures, res, n_views, size, camera_pos, light_pos, light_pow, images)
     switch (terms) {
         case 0: SVBRDF_LAUNCH_RENDER(0); break;
         case 1: SVBRDF_LAUNCH_RENDER(1); break;
