@@ -276,29 +276,46 @@ __global__ __launch_bounds__(kBlock) void textures_from_u8_kernel(const uint8_t 
 }
 
 // save_textures_th (src/svbrdf.py:185-207) with imwrite flags (src/imageio.py:
-// 52-72): [9, R, R] float32 -> dif / nom / spe [R, R, 3] and rgh [R, R] bytes.
+// 52-72) for one texel: [9] floats -> dif / nom / spe [3] and rgh bytes.
 // The normal is SvbrdfIO.reconstruct_normal (src/svbrdf.py:126-132): z from
 // clamp(x^2 + y^2, 0, 1), divided by the 2-norm summed as (x^2 + y^2) + z^2.
-__global__ __launch_bounds__(kBlock) void textures_to_u8_kernel(const float *__restrict__ tex, unsigned npix,
-                                                                uint8_t *__restrict__ dif, uint8_t *__restrict__ nom,
-                                                                uint8_t *__restrict__ spe, uint8_t *__restrict__ rgh) {
-    const unsigned p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= npix) return;
-    auto half = [](float t) { return (t + 1.0f) / 2.0f; };
+__device__ __forceinline__ void texel_to_u8(const float t[9], uint8_t dif[3], uint8_t nom[3], uint8_t spe[3],
+                                            uint8_t &rgh) {
+    auto half = [](float v) { return (v + 1.0f) / 2.0f; };
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        dif[3 * (size_t)p + c] = to_u8(clip(half(tex[(size_t)c * npix + p]), 0.0f, 1.0f));
-        spe[3 * (size_t)p + c] = to_u8(clip(half(tex[(size_t)(6 + c) * npix + p]), 0.0f, 1.0f));
+        dif[c] = to_u8(clip(half(t[c]), 0.0f, 1.0f));
+        spe[c] = to_u8(clip(half(t[6 + c]), 0.0f, 1.0f));
     }
-    rgh[p] = to_u8(clip(half(tex[(size_t)5 * npix + p]), 0.0f, 1.0f));
-    const float x = clip(tex[(size_t)3 * npix + p], -1.0f, 1.0f);
-    const float y = clip(tex[(size_t)4 * npix + p], -1.0f, 1.0f);
+    rgh = to_u8(clip(half(t[5]), 0.0f, 1.0f));
+    const float x = clip(t[3], -1.0f, 1.0f);
+    const float y = clip(t[4], -1.0f, 1.0f);
     const float xy = x * x + y * y;
     const float z = sqrtf(1.0f - clip(xy, 0.0f, 1.0f));
     const float norm = sqrtf(xy + z * z);
     const float n[3] = {x / norm, y / norm, z / norm};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) nom[3 * (size_t)p + c] = to_u8(half(clip(n[c], -1.0f, 1.0f)));
+    for (int c = 0; c < 3; ++c) nom[c] = to_u8(half(clip(n[c], -1.0f, 1.0f)));
+}
+
+// [9, R, R] float32 -> dif / nom / spe [R, R, 3] and rgh [R, R] bytes.
+__global__ __launch_bounds__(kBlock) void textures_to_u8_kernel(const float *__restrict__ tex, unsigned npix,
+                                                                uint8_t *__restrict__ dif, uint8_t *__restrict__ nom,
+                                                                uint8_t *__restrict__ spe, uint8_t *__restrict__ rgh) {
+    const unsigned p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    float t[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) t[c] = tex[(size_t)c * npix + p];
+    uint8_t d[3], n[3], sp[3], r;
+    texel_to_u8(t, d, n, sp, r);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dif[3 * (size_t)p + c] = d[c];
+        nom[3 * (size_t)p + c] = n[c];
+        spe[3 * (size_t)p + c] = sp[c];
+    }
+    rgh[p] = r;
 }
 
 // imwrite before the PNG encode (src/imageio.py:52-72), for n images of
@@ -319,6 +336,147 @@ __global__ __launch_bounds__(kBlock) void images_to_u8_kernel(const float *__res
                                                              : to_u8(clip(v, 0.0f, 1.0f));
     }
 }
+
+// ---------------------------------------------------------------------------
+// Four pixels per thread: the common layouts (3-channel images, pixel counts
+// divisible by 4, 4-byte aligned bytes and 16-byte aligned floats) move as
+// whole dwords and float4s instead of single bytes and floats.  The per-pixel
+// arithmetic is the functions above, so the results are the same bits.
+struct Px4 {
+    uint8_t b[12];   // 4 pixels x 3 channels, as stored
+};
+__device__ __forceinline__ Px4 load_px4(const uint8_t *base, size_t group) {
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(base) + 3 * group;
+    Px4 p;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = w[k];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) p.b[4 * k + i] = (uint8_t)(v >> (8 * i));
+    }
+    return p;
+}
+__device__ __forceinline__ void store_px4(uint8_t *base, size_t group, const uint8_t (&b)[12]) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(base) + 3 * group;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        w[k] = (uint32_t)b[4 * k] | ((uint32_t)b[4 * k + 1] << 8) | ((uint32_t)b[4 * k + 2] << 16) |
+               ((uint32_t)b[4 * k + 3] << 24);
+}
+__device__ __forceinline__ void store4(float *plane, size_t q, float a, float b, float c, float d) {
+    *reinterpret_cast<float4 *>(plane + q) = make_float4(a, b, c, d);
+}
+__device__ __forceinline__ float4 load4(const float *plane, size_t q) {
+    return *reinterpret_cast<const float4 *>(plane + q);
+}
+
+// images_from_u8_kernel for 3-channel images, npix % 4 == 0.
+__global__ __launch_bounds__(kBlock) void images_from_u8_x4_kernel(const uint8_t *__restrict__ in, size_t groups,
+                                                                   unsigned npix, int flag, float *__restrict__ out) {
+    const size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= groups) return;
+    const size_t p = 4 * g, img = p / npix, q = p - img * npix;   // the 4 pixels share one image
+    const Px4 x = load_px4(in, g);
+    if (flag == SVBRDF_IMREAD_ROUGH) {
+        store4(out + img * npix, q, decode_rough(x.b, 3), decode_rough(x.b + 3, 3), decode_rough(x.b + 6, 3),
+               decode_rough(x.b + 9, 3));
+    } else if (flag == SVBRDF_IMREAD_NORMAL) {
+        float n[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) decode_normal(x.b + 3 * k, n[k]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) store4(out + (img * 3 + c) * npix, q, n[0][c], n[1][c], n[2][c], n[3][c]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            store4(out + (img * 3 + c) * npix, q, unit(x.b[c]), unit(x.b[3 + c]), unit(x.b[6 + c]),
+                   unit(x.b[9 + c]));
+    }
+}
+
+// textures_from_u8_kernel, npix % 4 == 0.
+__global__ __launch_bounds__(kBlock) void textures_from_u8_x4_kernel(const uint8_t *__restrict__ dif,
+                                                                     const uint8_t *__restrict__ nom,
+                                                                     const uint8_t *__restrict__ spe,
+                                                                     const uint8_t *__restrict__ rgh,
+                                                                     int rgh_channels, unsigned npix,
+                                                                     float *__restrict__ tex) {
+    const unsigned g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= npix / 4) return;
+    const unsigned q = 4 * g;
+    const Px4 d = load_px4(dif, g), s = load_px4(spe, g), n = load_px4(nom, g);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        store4(tex + (size_t)c * npix, q, unit(d.b[c]) * 2.0f - 1.0f, unit(d.b[3 + c]) * 2.0f - 1.0f,
+               unit(d.b[6 + c]) * 2.0f - 1.0f, unit(d.b[9 + c]) * 2.0f - 1.0f);
+        store4(tex + (size_t)(6 + c) * npix, q, unit(s.b[c]) * 2.0f - 1.0f, unit(s.b[3 + c]) * 2.0f - 1.0f,
+               unit(s.b[6 + c]) * 2.0f - 1.0f, unit(s.b[9 + c]) * 2.0f - 1.0f);
+    }
+    float r[4];
+    if (rgh_channels == 3) {
+        const Px4 x = load_px4(rgh, g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = decode_rough(x.b + 3 * k, 3) * 2.0f - 1.0f;
+    } else {
+        const uint32_t w = reinterpret_cast<const uint32_t *>(rgh)[g];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = unit((uint8_t)(w >> (8 * k))) * 2.0f - 1.0f;
+    }
+    store4(tex + (size_t)5 * npix, q, r[0], r[1], r[2], r[3]);
+    float nv[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) decode_normal(n.b + 3 * k, nv[k]);
+    store4(tex + (size_t)3 * npix, q, nv[0][0], nv[1][0], nv[2][0], nv[3][0]);
+    store4(tex + (size_t)4 * npix, q, nv[0][1], nv[1][1], nv[2][1], nv[3][1]);
+}
+
+// textures_to_u8_kernel, npix % 4 == 0.
+__global__ __launch_bounds__(kBlock) void textures_to_u8_x4_kernel(const float *__restrict__ tex, unsigned npix,
+                                                                   uint8_t *__restrict__ dif, uint8_t *__restrict__ nom,
+                                                                   uint8_t *__restrict__ spe, uint8_t *__restrict__ rgh) {
+    const unsigned g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= npix / 4) return;
+    const unsigned q = 4 * g;
+    float4 v[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) v[c] = load4(tex + (size_t)c * npix, q);
+    uint8_t d[12], n[12], s[12], r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float t[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = k == 0 ? v[c].x : k == 1 ? v[c].y : k == 2 ? v[c].z : v[c].w;
+        texel_to_u8(t, d + 3 * k, n + 3 * k, s + 3 * k, r[k]);
+    }
+    store_px4(dif, g, d);
+    store_px4(nom, g, n);
+    store_px4(spe, g, s);
+    reinterpret_cast<uint32_t *>(rgh)[g] =
+        (uint32_t)r[0] | ((uint32_t)r[1] << 8) | ((uint32_t)r[2] << 16) | ((uint32_t)r[3] << 24);
+}
+
+// images_to_u8_kernel for 3-channel images, npix % 4 == 0.
+__global__ __launch_bounds__(kBlock) void images_to_u8_x4_kernel(const float *__restrict__ in, size_t groups,
+                                                                 unsigned npix, int flag, uint8_t *__restrict__ out) {
+    const size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= groups) return;
+    const size_t p = 4 * g, img = p / npix, q = p - img * npix;
+    float4 v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = load4(in + (img * 3 + c) * npix, q);
+    uint8_t b[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float x = k == 0 ? v[c].x : k == 1 ? v[c].y : k == 2 ? v[c].z : v[c].w;
+            b[3 * k + c] = flag == SVBRDF_IMREAD_NORMAL ? to_u8((clip(x, -1.0f, 1.0f) + 1.0f) / 2.0f)
+                                                        : to_u8(clip(x, 0.0f, 1.0f));
+        }
+    store_px4(out, g, b);
+}
+
+bool aligned(const void *p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
 
 unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -374,8 +532,12 @@ int svbrdf_images_from_u8(const unsigned char *images, int n, int height, int wi
     if (rc) return rc;
     const unsigned npix = (unsigned)height * (unsigned)width;
     const size_t total = (size_t)n * npix;
-    hipLaunchKernelGGL(images_from_u8_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, (hipStream_t)stream, images,
-                       total, npix, channels, flag, out);
+    if (channels == 3 && npix % 4 == 0 && aligned(images, 4) && aligned(out, 16))
+        hipLaunchKernelGGL(images_from_u8_x4_kernel, dim3(blocks_for(total / 4)), dim3(kBlock), 0,
+                           (hipStream_t)stream, images, total / 4, npix, flag, out);
+    else
+        hipLaunchKernelGGL(images_from_u8_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, (hipStream_t)stream,
+                           images, total, npix, channels, flag, out);
     return check_launch(what);
 }
 
@@ -387,8 +549,13 @@ int svbrdf_textures_from_u8(const unsigned char *dif, const unsigned char *nom, 
         return fail(SVBRDF_EINVAL, "%s: roughness map must have 1 or 3 channels, got %d", what, rgh_channels);
     if (res < 1 || res > kMaxRes) return fail(SVBRDF_ESHAPE, "%s: res=%d outside [1, %d]", what, res, kMaxRes);
     const unsigned npix = (unsigned)res * (unsigned)res;
-    hipLaunchKernelGGL(textures_from_u8_kernel, dim3(blocks_for(npix)), dim3(kBlock), 0, (hipStream_t)stream, dif,
-                       nom, spe, rgh, rgh_channels, npix, textures);
+    if (npix % 4 == 0 && aligned(dif, 4) && aligned(nom, 4) && aligned(spe, 4) && aligned(rgh, 4) &&
+        aligned(textures, 16))
+        hipLaunchKernelGGL(textures_from_u8_x4_kernel, dim3(blocks_for(npix / 4)), dim3(kBlock), 0,
+                           (hipStream_t)stream, dif, nom, spe, rgh, rgh_channels, npix, textures);
+    else
+        hipLaunchKernelGGL(textures_from_u8_kernel, dim3(blocks_for(npix)), dim3(kBlock), 0, (hipStream_t)stream,
+                           dif, nom, spe, rgh, rgh_channels, npix, textures);
     return check_launch(what);
 }
 
@@ -398,8 +565,13 @@ int svbrdf_textures_to_u8(const float *textures, int res, unsigned char *dif, un
     if (!textures || !dif || !nom || !spe || !rgh) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
     if (res < 1 || res > kMaxRes) return fail(SVBRDF_ESHAPE, "%s: res=%d outside [1, %d]", what, res, kMaxRes);
     const unsigned npix = (unsigned)res * (unsigned)res;
-    hipLaunchKernelGGL(textures_to_u8_kernel, dim3(blocks_for(npix)), dim3(kBlock), 0, (hipStream_t)stream, textures,
-                       npix, dif, nom, spe, rgh);
+    if (npix % 4 == 0 && aligned(textures, 16) && aligned(dif, 4) && aligned(nom, 4) && aligned(spe, 4) &&
+        aligned(rgh, 4))
+        hipLaunchKernelGGL(textures_to_u8_x4_kernel, dim3(blocks_for(npix / 4)), dim3(kBlock), 0,
+                           (hipStream_t)stream, textures, npix, dif, nom, spe, rgh);
+    else
+        hipLaunchKernelGGL(textures_to_u8_kernel, dim3(blocks_for(npix)), dim3(kBlock), 0, (hipStream_t)stream,
+                           textures, npix, dif, nom, spe, rgh);
     return check_launch(what);
 }
 
@@ -412,8 +584,12 @@ int svbrdf_images_to_u8(const float *images, int n, int height, int width, int c
         return fail(SVBRDF_EINVAL, "%s: a roughness image is written with 1 channel", what);
     const unsigned npix = (unsigned)height * (unsigned)width;
     const size_t total = (size_t)n * npix;
-    hipLaunchKernelGGL(images_to_u8_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, (hipStream_t)stream, images,
-                       total, npix, channels, flag, out);
+    if (channels == 3 && npix % 4 == 0 && aligned(images, 16) && aligned(out, 4))
+        hipLaunchKernelGGL(images_to_u8_x4_kernel, dim3(blocks_for(total / 4)), dim3(kBlock), 0,
+                           (hipStream_t)stream, images, total / 4, npix, flag, out);
+    else
+        hipLaunchKernelGGL(images_to_u8_kernel, dim3(blocks_for(total)), dim3(kBlock), 0, (hipStream_t)stream,
+                           images, total, npix, channels, flag, out);
     return check_launch(what);
 }
 

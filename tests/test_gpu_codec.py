@@ -37,13 +37,29 @@ def test_resize_matches_oracle_bit_for_bit(pkg, shape, dst):
     np.testing.assert_array_equal(out, ref)
 
 
+def on_device(a, misalign=False):
+    """A device copy; misalign=True places it one byte (or float) past an
+    aligned address, so that the library takes its one-pixel-per-thread path
+    (four pixels per thread need 4-byte aligned bytes and 16-byte aligned
+    floats and a pixel count divisible by 4)."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if not misalign:
+        return t.to(DEV)
+    buf = torch.empty(t.numel() + 1, dtype=t.dtype, device=DEV)
+    v = buf[1:].view(t.shape)
+    v.copy_(t)
+    return v
+
+
+# (17, 29) and the misaligned copies: one pixel per thread; (16, 20): four
+@pytest.mark.parametrize("hw,misalign", [((17, 29), False), ((16, 20), False), ((16, 20), True)])
 @pytest.mark.parametrize("flag,channels", [(None, 3), (None, 1), ("srgb", 3), ("rough", 3), ("rough", 1),
                                            ("normal", 3), (None, 4)])
-def test_decode_flags_match_oracle(pkg, flag, channels):
+def test_decode_flags_match_oracle(pkg, flag, channels, hw, misalign):
     from svbrdf_amd import imageio as io
     rng = np.random.default_rng(channels)
-    ims = rng.integers(0, 256, (3, 17, 29, channels), dtype=np.uint8)
-    out = io.decode(torch.from_numpy(ims).to(DEV), flag).cpu().numpy()
+    ims = rng.integers(0, 256, (3, *hw, channels), dtype=np.uint8)
+    out = io.decode(on_device(ims, misalign), flag).cpu().numpy()
     for k in range(3):
         src = ims[k] if channels > 1 else ims[k, :, :, 0]
         ref = C.imread_u8(src, flag)
@@ -51,13 +67,14 @@ def test_decode_flags_match_oracle(pkg, flag, channels):
         np.testing.assert_array_equal(out[k], ref)
 
 
+@pytest.mark.parametrize("hw,misalign", [((19, 23), False), ((16, 12), False), ((16, 12), True)])
 @pytest.mark.parametrize("flag,channels", [("srgb", 3), ("rough", 1), ("normal", 3), (None, 3)])
-def test_encode_flags_match_oracle(pkg, flag, channels):
+def test_encode_flags_match_oracle(pkg, flag, channels, hw, misalign):
     from svbrdf_amd import imageio as io
     rng = np.random.default_rng(10 + channels)
-    x = rng.uniform(-1.3, 1.3, (2, channels, 19, 23)).astype(np.float32)
+    x = rng.uniform(-1.3, 1.3, (2, channels, *hw)).astype(np.float32)
     x[0, 0, 0, :4] = [0.0, 1.0, 0.5, -0.0]
-    out = io.encode(torch.from_numpy(x).to(DEV), flag).cpu().numpy()
+    out = io.encode(on_device(x, misalign), flag).cpu().numpy()
     for k in range(2):
         v = x[k].transpose(1, 2, 0)
         if flag == "normal":
@@ -67,23 +84,24 @@ def test_encode_flags_match_oracle(pkg, flag, channels):
         np.testing.assert_array_equal(out[k], ref)
 
 
-@pytest.mark.parametrize("res,rgh_channels", [(64, 1), (64, 3), (33, 3), (1, 1)])
-def test_textures_from_u8_matches_oracle(pkg, res, rgh_channels):
+@pytest.mark.parametrize("res,rgh_channels,misalign", [(64, 1, False), (64, 3, False), (33, 3, False), (1, 1, False),
+                                                        (64, 3, True)])
+def test_textures_from_u8_matches_oracle(pkg, res, rgh_channels, misalign):
     from svbrdf_amd.svbrdf_io import textures_from_maps_u8
     rng = np.random.default_rng(res + rgh_channels)
     dif, nom, spe = (rng.integers(0, 256, (res, res, 3), dtype=np.uint8) for _ in range(3))
     rgh = rng.integers(0, 256, (res, res, 3) if rgh_channels == 3 else (res, res), dtype=np.uint8)
-    t = textures_from_maps_u8(*(torch.from_numpy(m).to(DEV) for m in (dif, nom, spe, rgh)), res)
+    t = textures_from_maps_u8(*(on_device(m, misalign) for m in (dif, nom, spe, rgh)), res)
     np.testing.assert_array_equal(t[0].cpu().numpy(), C.load_textures(dif, nom, spe, rgh, res))
 
 
-@pytest.mark.parametrize("res", [64, 33, 1])
-def test_textures_to_u8_matches_oracle(pkg, res):
+@pytest.mark.parametrize("res,misalign", [(64, False), (33, False), (1, False), (64, True)])
+def test_textures_to_u8_matches_oracle(pkg, res, misalign):
     from svbrdf_amd.svbrdf_io import maps_u8_from_textures
     rng = np.random.default_rng(100 + res)
     tex = rng.uniform(-1.2, 1.2, (9, res, res)).astype(np.float32)
     tex[3:5, 0, 0] = [0.0, 0.0]              # flat normal
-    out = [m.cpu().numpy() for m in maps_u8_from_textures(torch.from_numpy(tex).to(DEV)[None])]
+    out = [m.cpu().numpy() for m in maps_u8_from_textures(on_device(tex, misalign)[None])]
     ref = C.save_textures(tex)
     for name, a, b in zip(("dif", "nom", "spe", "rgh"), out, ref):
         np.testing.assert_array_equal(a, b, err_msg=name)
