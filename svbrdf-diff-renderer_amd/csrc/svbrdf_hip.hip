@@ -758,17 +758,17 @@ __device__ __forceinline__ float4 *partial_buffer(void *ws, unsigned nb, unsigne
 // then the wave and the block combine in index order.  Every reader of a
 // step's partials (the next step's block 0, the flush, the light finalize)
 // uses this routine, so they agree to the bit.  Result valid in thread 0.
-template <int NQ>
+template <int NQ, int BLOCK = kBlock>
 __device__ void reduce_partials(const float4 *__restrict__ parts, unsigned n, double (&tot)[NQ],
-                                double (*lds)[kBlock / 64]) {
+                                double (*lds)[BLOCK / 64]) {
     double acc[NQ];
 #pragma unroll
     for (int c = 0; c < NQ; ++c) acc[c] = 0.0;
-    for (unsigned b0 = threadIdx.x; b0 < n; b0 += 4 * kBlock) {
+    for (unsigned b0 = threadIdx.x; b0 < n; b0 += 4 * BLOCK) {
         float4 q[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const unsigned b = b0 + u * kBlock;
+            const unsigned b = b0 + u * BLOCK;
             q[u] = b < n ? parts[b] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
@@ -790,7 +790,7 @@ __device__ void reduce_partials(const float4 *__restrict__ parts, unsigned n, do
     for (int c = 0; c < NQ; ++c) {
         tot[c] = 0.0;
         if (threadIdx.x == 0)
-            for (int w = 0; w < kBlock / 64; ++w) tot[c] += lds[c][w];
+            for (int w = 0; w < BLOCK / 64; ++w) tot[c] += lds[c][w];
     }
     __syncthreads();
 }
@@ -882,20 +882,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     }
 }
 
-// One block, after a light-optimising step: reduces that step's partials in
-// the fixed order of reduce_partials, records the loss, takes the
+// One block of kFinalizeBlock threads, after a light-optimising step:
+// reduces that step's partials in the fixed order of reduce_partials (at this
+// block size: the light path's only reader), records the loss, takes the
 // light-power Adam step (one param group with the maps, src/svbrdf.py:50)
-// and advances the state.
+// and advances the state.  The next step needs the new light power in every
+// block, so this sits between two step launches; 1,024 threads take the
+// 4,096 partials of a 1024^2 step in one round of loads.
+constexpr int kFinalizeBlock = 1024;
 template <bool OPT_LIGHT>
-__global__ __launch_bounds__(kBlock) void finalize_step_kernel(void *ws, unsigned n_partials, StepConsts k,
-                                                              float *__restrict__ lpow, svbrdf_step_state *st,
-                                                              float *loss_history, int history_len) {
+__global__ __launch_bounds__(kFinalizeBlock) void finalize_step_kernel(void *ws, unsigned n_partials, StepConsts k,
+                                                                      float *__restrict__ lpow,
+                                                                      svbrdf_step_state *st, float *loss_history,
+                                                                      int history_len) {
     constexpr int NQ = OPT_LIGHT ? 4 : 1;
-    __shared__ double lds[NQ][kBlock / 64];
+    __shared__ double lds[NQ][kFinalizeBlock / 64];
     const float step_f = st->step;
     const unsigned t = (unsigned)step_f;
     double tot[NQ];
-    reduce_partials<NQ>(partial_buffer(ws, n_partials, t & 1u), n_partials, tot, lds);
+    reduce_partials<NQ, kFinalizeBlock>(partial_buffer(ws, n_partials, t & 1u), n_partials, tot, lds);
     if (threadIdx.x != 0) return;
     record_loss(st, t + 1, tot[0], k.count, loss_history, history_len);
     if constexpr (OPT_LIGHT) {
@@ -1245,8 +1250,8 @@ int svbrdf_fused_step(float *textures, float *exp_avg, float *exp_avg_sq, const 
 #undef SVBRDF_LAUNCH_STEP
     if ((rc = check_launch("svbrdf_fused_step"))) return rc;
     if (optim_light) {
-        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kBlock), 0, s, workspace, nb, consts, light_pow,
-                           st, loss_history, history_len);
+        hipLaunchKernelGGL(finalize_step_kernel<true>, dim3(1), dim3(kFinalizeBlock), 0, s, workspace, nb, consts,
+                           light_pow, st, loss_history, history_len);
     }
     return check_launch("svbrdf_fused_step");
 }
