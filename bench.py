@@ -28,6 +28,7 @@ Printed JSON line (one):
                     cpu_baseline_256 the same at 256^2 (config 0)
   forward           forward-render throughput at the same shape (config 1)
   adjoint           the render adjoint alone (dL/dimages -> dL/dtextures, 180 B/pixel)
+  optim_light       the fused step with the light power optimised as well
 
   batched           config 3: 64 materials of uneven size in one persistent
                     launch (autotuned), against a per-material loop
@@ -301,6 +302,24 @@ def batch_leg(pkg, dev, steps=20, warmup=3):
             "per_material_loop": {"ms_per_step": round(ms_loop, 4), "what": "FusedStep per material, 64 steps "
                                   "(one launch each) in one HIP graph"},
             "speedup_vs_loop": round(ms_loop / ms, 3)}
+
+
+def light_leg(pkg, dev, res, views, per=100, n=20):
+    """The fused step with the light power optimised too (src/svbrdf.py:47-50,
+    58-59): one step launch plus a one-block finalize per iteration, replayed
+    from graphs of `per` iterations; median us per iteration."""
+    prob = make_problem(pkg, res, views, dev)
+    r = prob["renderer"]
+    r.update_light(r.light_pow.detach().to(torch.float32).reshape(3).clone().contiguous())
+    fs = pkg.FusedStep(r, prob["tex"], prob["target"], lr=0.01, loss="log_l2", optim_light=True,
+                       own_textures=True)
+    g = fs._capture_graph(per)
+    g.replay()
+    sp = replay_spread(lambda k: g.replay(), per, n)
+    nbytes = algorithmic_bytes_step(res, views)
+    return {"us_per_step": sp["median"], "iqr": sp["iqr"], "value": round(1e6 / sp["median"], 1), "unit": "it/s",
+            "frac": round(nbytes / (sp["median"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "light_pow_after": [round(float(x), 3) for x in fs.light_pow.cpu()]}
 
 
 def codec_leg(pkg, dev, res=1024, views=9, iters=20):
@@ -604,6 +623,7 @@ def main():
                       "achieved_GBs": round(bb / (bms * 1e-3) / 1e9, 1),
                       "frac": round(bb / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    out["optim_light"] = light_leg(pkg, dev, res, views)
     if not args.no_batch:
         out["batched"] = batch_leg(pkg, dev)
     if not args.no_compose:
