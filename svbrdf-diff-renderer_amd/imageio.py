@@ -2,9 +2,10 @@
 work on the GPU.
 
 The reference reads and writes PNG files with cv2 (BGR order in memory) and
-does the per-pixel work in numpy on the host.  Here the PNG container --
-inflate / deflate and the scanline filters, sequential byte work -- is
-Pillow's, on the host; the arrays keep the file's RGB order, so the
+does the per-pixel work in numpy on the host.  Here the PNG container stays
+on the host: files are read with Pillow and written by :func:`encode_png`
+(Sub-filtered scanlines deflated in parallel pieces on a thread pool, zlib
+level 1 as cv2's default); the arrays keep the file's RGB order, so the
 reference's BGR<->RGB swaps cancel.  Everything per pixel runs in
 libsvbrdf_hip.so:
 
@@ -22,7 +23,11 @@ libsvbrdf_hip.so:
 Only 8-bit PNGs are handled (the reference's data are 8-bit; its uint16
 branch, src/imageio.py:19-20, has no 16-bit RGB counterpart in Pillow).
 """
+import concurrent.futures
+import os
 import pathlib
+import struct
+import zlib
 
 import numpy as np
 import torch
@@ -53,10 +58,71 @@ def read_png(filename):
         return np.asarray(im, dtype=np.uint8).copy()
 
 
+# zlib level 1: OpenCV's default speed setting for cv2.imwrite's PNGs
+# (IMWRITE_PNG_COMPRESSION), which the reference writes with; the pixels are
+# the same at any level.
+PNG_COMPRESS_LEVEL = 1
+_PNG_COLOR_TYPE = {1: 0, 2: 4, 3: 2, 4: 6}      # channels -> grey, grey+alpha, RGB, RGBA
+_CHUNK_BYTES = 1 << 18                         # deflate work unit
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        _POOL = concurrent.futures.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1))
+    return _POOL
+
+
+def _png_chunk(kind, data):
+    return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+
+def encode_png(arr):
+    """PNG bytes of [H, W] or [H, W, 1..4] uint8.  Every scanline uses the Sub
+    filter (each byte minus the one a pixel to its left, mod 256: one numpy
+    subtraction for the whole image), and the filtered rows are deflated in
+    independent pieces on a thread pool (zlib releases the GIL), joined as one
+    zlib stream: full flushes between the pieces, one header, and the Adler-32
+    of the whole stream -- the parallel-deflate layout any inflater reads."""
+    arr = np.ascontiguousarray(arr, dtype=np.uint8)
+    if arr.ndim == 2:
+        arr = arr[:, :, None]
+    if arr.ndim != 3 or arr.shape[2] not in _PNG_COLOR_TYPE or arr.shape[0] < 1 or arr.shape[1] < 1:
+        raise ValueError(f"cannot write an 8-bit PNG of shape {arr.shape}")
+    h, w, c = arr.shape
+    rows = arr.reshape(h, w * c)
+    raw = np.empty((h, w * c + 1), dtype=np.uint8)
+    raw[:, 0] = 1                                   # filter type 1 (Sub)
+    raw[:, 1:c + 1] = rows[:, :c]
+    np.subtract(rows[:, c:], rows[:, :-c], out=raw[:, c + 1:])
+    data = memoryview(raw.reshape(-1))
+    step = max(1, _CHUNK_BYTES // raw.shape[1]) * raw.shape[1]
+    pieces = [data[i:i + step] for i in range(0, len(data), step)]
+
+    def deflate(k):
+        z = zlib.compressobj(PNG_COMPRESS_LEVEL, zlib.DEFLATED, -15)
+        out = z.compress(pieces[k])
+        return out + z.flush(zlib.Z_FINISH if k == len(pieces) - 1 else zlib.Z_FULL_FLUSH)
+    if len(pieces) == 1:
+        body = [deflate(0)]
+    else:
+        body = list(_pool().map(deflate, range(len(pieces))))
+    stream = b"\x78\x01" + b"".join(body) + struct.pack(">I", zlib.adler32(data) & 0xFFFFFFFF)
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, _PNG_COLOR_TYPE[c], 0, 0, 0)
+    return b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", ihdr) + _png_chunk(b"IDAT", stream) + _png_chunk(b"IEND", b"")
+
+
 def write_png(filename, arr):
     """Writes [H, W] or [H, W, 3 | 4] uint8 (RGB order) as an 8-bit PNG."""
-    arr = np.ascontiguousarray(arr, dtype=np.uint8)
-    Image.fromarray(arr).save(str(filename), format="PNG")
+    pathlib.Path(filename).write_bytes(encode_png(arr))
+
+
+def write_pngs(items):
+    """Writes several (filename, array) PNGs; each one's deflate runs on the
+    thread pool.  Returns when all are on disk."""
+    for f, a in items:
+        write_png(f, a)
 
 
 def _device(device):
@@ -161,20 +227,31 @@ def imconcat(im_list, size=(2, 2)):
     return np.concatenate(rows, axis=0)
 
 
+def montage9(ims):
+    """The 3 x 3 montage of img9to1 from the 9 stored images (bytes)."""
+    return imconcat(list(ims), (3, 3))
+
+
+def montage_tex(nom, dif, spe, rgh):
+    """The 2 x 2 montage of tex4to1 from the stored maps (bytes); a grey
+    roughness map is made 3-channel."""
+    if rgh.ndim == 2:
+        rgh = np.repeat(rgh[:, :, None], 3, axis=2)
+    return imconcat([nom, dif, spe, rgh])
+
+
 def img9to1(folder):
     """src/imageio.py:93-99: the 9 views 00..08.png as a 3 x 3 montage all.png.
     The reference passes every pixel through imread's u / 255 and imwrite's
     (v * 255).astype(uint8); in float32 that returns each of the 256 levels
     unchanged, so the montage holds the stored bytes."""
     folder = pathlib.Path(folder)
-    write_png(folder / "all.png", imconcat([read_png(folder / f"{i:02d}.png") for i in range(9)], (3, 3)))
+    write_png(folder / "all.png", montage9(read_png(folder / f"{i:02d}.png") for i in range(9)))
 
 
 def tex4to1(folder):
     """src/imageio.py:102-113: normal, diffuse, specular, roughness (grey made
     3-channel) as a 2 x 2 montage tex.png (stored bytes, as in img9to1)."""
     folder = pathlib.Path(folder)
-    ims = [read_png(folder / f"{n}.png") for n in ("nom", "dif", "spe", "rgh")]
-    if ims[3].ndim == 2:
-        ims[3] = np.repeat(ims[3][:, :, None], 3, axis=2)
-    write_png(folder / "tex.png", imconcat(ims))
+    maps = [read_png(folder / f"{n}.png") for n in ("nom", "dif", "spe", "rgh")]
+    write_png(folder / "tex.png", montage_tex(*maps))

@@ -25,7 +25,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .imageio import imresize, read_png, tex4to1, img9to1, to_device_u8, write_png, decode, encode
+from .imageio import (decode, encode, img9to1, imresize, montage9, montage_tex, read_png, to_device_u8,
+                      write_pngs)
 
 MAP_FILES = {"normal": "nom.png", "diffuse": "dif.png", "specular": "spe.png", "roughness": "rgh.png"}
 
@@ -145,10 +146,11 @@ class SvbrdfIO:
         """src/svbrdf.py:185-207: nom / dif / spe / rgh.png and tex.png."""
         d = pathlib.Path(textures_dir)
         d.mkdir(parents=True, exist_ok=True)
-        dif, nom, spe, rgh = maps_u8_from_textures(textures_th)
-        for name, m in (("nom.png", nom), ("dif.png", dif), ("spe.png", spe), ("rgh.png", rgh)):
-            write_png(d / name, m.cpu().numpy())
-        tex4to1(d)
+        dif, nom, spe, rgh = (m.cpu().numpy() for m in maps_u8_from_textures(textures_th))
+        # the montage of tex4to1 from the same bytes the map files hold; all
+        # five files encoded at once
+        write_pngs([(d / "nom.png", nom), (d / "dif.png", dif), (d / "spe.png", spe), (d / "rgh.png", rgh),
+                    (d / "tex.png", montage_tex(nom, dif, spe, rgh))])
 
     def load_images_th(self, images_dir, res=256):
         """src/svbrdf.py:209-221 -> [n_of_imgs, 3, res, res] in [0, 1]."""
@@ -170,7 +172,10 @@ class SvbrdfIO:
         d = pathlib.Path(images_dir)
         d.mkdir(parents=True, exist_ok=True)
         u = encode(images_th, "srgb").cpu().numpy()
-        for k, i in enumerate(self.idx):
-            write_png(d / f"{i:02d}.png", u[k])
-        if self.n_of_imgs == 9:
+        items = [(d / f"{i:02d}.png", u[k]) for k, i in enumerate(self.idx)]
+        if self.n_of_imgs == 9 and list(self.idx) == list(range(9)):
+            # img9to1's montage of 00..08.png, from the bytes just encoded
+            items.append((d / "all.png", montage9(u)))
+        write_pngs(items)
+        if self.n_of_imgs == 9 and list(self.idx) != list(range(9)):
             img9to1(d)

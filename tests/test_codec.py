@@ -165,3 +165,45 @@ def test_io_refuses_cpu(pkg):
     from svbrdf_amd import imageio as io
     with pytest.raises(RuntimeError, match="GPU"):
         io.to_device_u8(np.zeros((2, 2, 3), np.uint8), "cpu")
+
+
+def test_png_writes_and_montages_are_the_stored_bytes(pkg, tmp_path):
+    """write_pngs (several files on a thread pool, zlib level 1) stores exactly
+    the given bytes, and the in-memory montages equal img9to1 / tex4to1, which
+    read the files back (src/imageio.py:93-113)."""
+    from svbrdf_amd import imageio as io
+    rng = np.random.default_rng(3)
+    views = [rng.integers(0, 256, (20, 24, 3), dtype=np.uint8) for _ in range(9)]
+    io.write_pngs([(tmp_path / f"{i:02d}.png", v) for i, v in enumerate(views)])
+    for i, v in enumerate(views):
+        np.testing.assert_array_equal(io.read_png(tmp_path / f"{i:02d}.png"), v)
+    io.img9to1(tmp_path)
+    np.testing.assert_array_equal(io.read_png(tmp_path / "all.png"), io.montage9(views))
+    assert io.montage9(views).shape == (60, 72, 3)
+    maps = {n: rng.integers(0, 256, (16, 16, 3), dtype=np.uint8) for n in ("nom", "dif", "spe")}
+    maps["rgh"] = rng.integers(0, 256, (16, 16), dtype=np.uint8)
+    io.write_pngs([(tmp_path / f"{n}.png", m) for n, m in maps.items()])
+    io.tex4to1(tmp_path)
+    tex = io.read_png(tmp_path / "tex.png")
+    np.testing.assert_array_equal(tex, io.montage_tex(maps["nom"], maps["dif"], maps["spe"], maps["rgh"]))
+    np.testing.assert_array_equal(tex[16:, 16:, 1], maps["rgh"])     # grey roughness, bottom right
+    io.write_pngs([])
+    io.write_pngs([(tmp_path / "one.png", maps["rgh"])])
+    np.testing.assert_array_equal(io.read_png(tmp_path / "one.png"), maps["rgh"])
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 5, 3), (37, 53, 3), (16, 16, 4), (20, 7, 2), (300, 200),
+                                   (700, 300, 3)])
+def test_png_encoder_round_trips_through_pillow(pkg, tmp_path, shape):
+    """The parallel-deflate PNG writer (Sub filter, independent deflate pieces
+    joined by full flushes) decodes, with Pillow, to the bytes it was given --
+    one piece for small images, several above 256 KiB of scanlines."""
+    from svbrdf_amd import imageio as io
+    rng = np.random.default_rng(sum(shape))
+    a = rng.integers(0, 256, shape, dtype=np.uint8)
+    if len(shape) == 3 and shape[0] > 100:
+        a[:, :, 0] = np.arange(shape[1], dtype=np.uint8)[None]      # some structure to compress
+    io.write_png(tmp_path / "x.png", a)
+    np.testing.assert_array_equal(io.read_png(tmp_path / "x.png"), a)
+    with pytest.raises(ValueError):
+        io.encode_png(np.zeros((4, 4, 5), np.uint8))
