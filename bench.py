@@ -37,6 +37,8 @@ Printed JSON line (one):
                     tensors, Lanczos-4 resize, stage transition) at the same shape
   materialgan       the MaterialGAN caller: generator -> render -> feature loss ->
                     latent Adam per iteration at 256^2 x 9, and the render share
+  driver            scripts.render + scripts.optim_perpixel end to end (1000 epochs
+                    with the reference's PNG / plot side outputs), wall clock
 
     python bench.py [--steps K] [--warmup W] [--res 1024] [--views 9]
                     [--no-cpu] [--no-torch] [--no-batch] [--no-compose] [--no-codec]
@@ -322,6 +324,51 @@ def light_leg(pkg, dev, res, views, per=100, n=20):
             "light_pow_after": [round(float(x), 3) for x in fs.light_pow.cpu()]}
 
 
+def driver_leg(pkg, dev, res=1024, epochs=1000):
+    """The reference's per-pixel workflow end to end (src/scripts.py:31-41,
+    67-97; run.py:37's 1024^2 / lr 0.01 / 1000 epochs from random maps): a
+    JSON configuration and 9 synthetic views on disk, render the targets,
+    then optim_perpixel with every side output the reference writes (maps,
+    re-renders and the loss plot after epoch 1, every 100th and the last).
+    Wall-clock seconds, files written, and the optimisation loop alone."""
+    import tempfile
+    syn = pkg.synthetic
+    cam, light = syn.flash_grid(n_views=9)
+    with tempfile.TemporaryDirectory() as tmp:
+        tmp = pathlib.Path(tmp)
+        cfg = tmp / "cfg.json"
+        cfg.write_text(json.dumps({"reference_dir": "ref", "target_dir": "target", "optimize_dir": "opt",
+                                   "rerender_dir": "rr", "im_size": float(syn.PLANE_SIZE_CM), "idx": list(range(9)),
+                                   "camera_pos": cam.tolist(), "light_pos": light.tolist(),
+                                   "light_pow": syn.light_power().tolist()}))
+        sio = pkg.SvbrdfIO(cfg, dev)
+        sio.save_textures_th(torch.tensor(syn.ground_truth_textures(res, seed=0), device=dev)[None],
+                             sio.reference_dir)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pkg.scripts.render(cfg, res)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        opt = pkg.scripts.optim_perpixel(cfg, res, 0.01, epochs, "random")
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        files = sum(1 for f in tmp.rglob("*") if f.is_file())
+        o2 = pkg.SvbrdfOptim(dev, opt.renderer_obj)
+        o2.init_from_randn()
+        o2.load_targets(opt.targets)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        o2.optim(epochs, 0.01)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+    return {"workload": f"scripts.render + scripts.optim_perpixel at {res}^2 x 9, lr 0.01, {epochs} epochs from "
+                        "random maps, with the reference's side outputs (PNG maps, re-renders, loss plot)",
+            "render_s": round(t1 - t0, 3), "optim_perpixel_s": round(t2 - t1, 3),
+            "epochs_per_s": round(epochs / (t2 - t1), 1), "files_written": files,
+            "loss_first": opt.loss_image_list[0], "loss_last": opt.loss_image_list[-1],
+            "loop_alone_s": round(t4 - t3, 4)}
+
+
 def codec_leg(pkg, dev, res=1024, views=9, iters=20):
     """The map / target codec around the path (src/imageio.py, src/svbrdf.py:
     168-237) at the bench shape: bytes moved / time, against 8 TB/s."""
@@ -512,6 +559,8 @@ def main():
     ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (config 4)")
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
+    ap.add_argument("--no-driver", action="store_true",
+                    help="skip the end-to-end optim_perpixel leg (files in a temporary directory)")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--no-eager-probes", action="store_true",
                     help="skip the single-launch and per-iteration-sync timings (so that a profiler's kernel "
@@ -632,6 +681,8 @@ def main():
         out["codec"] = codec_leg(pkg, dev, res, views)
     if not args.no_materialgan:
         out["materialgan"] = materialgan_leg(pkg, dev)
+    if not args.no_driver:
+        out["driver"] = driver_leg(pkg, dev, res)
     if not args.no_torch:
         try:
             tb = torch_gpu_baseline(prob, res, views, dev)
