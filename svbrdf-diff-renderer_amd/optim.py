@@ -371,15 +371,17 @@ class SvbrdfOptim:
 
     @staticmethod
     def save_loss(losses, labels, save_dir, N):
-        """src/optimization.py:35-43: log(1 + loss) curves over N epochs."""
-        import matplotlib
-        matplotlib.use("Agg")
-        import matplotlib.pyplot as plt
-        plt.figure(figsize=(8, 4))
+        """src/optimization.py:35-43: log(1 + loss) curves over N epochs.  Drawn
+        on a standalone Agg figure, so pyplot's global state and the caller's
+        backend are left alone."""
+        from matplotlib.backends.backend_agg import FigureCanvasAgg
+        from matplotlib.figure import Figure
+        fig = Figure(figsize=(8, 4))
+        FigureCanvasAgg(fig)
+        ax = fig.add_subplot()
         for loss, label in zip(losses, labels):
-            plt.plot(np.log1p(np.asarray(loss, dtype=np.float64)), label=label)
-        plt.xlim(0, N)
-        plt.legend()
-        plt.title("log(1+loss)")
-        plt.savefig(save_dir)
-        plt.close()
+            ax.plot(np.log1p(np.asarray(loss, dtype=np.float64)), label=label)
+        ax.set_xlim(0, N)
+        ax.legend()
+        ax.set_title("log(1+loss)")
+        fig.savefig(save_dir)

@@ -138,3 +138,16 @@ def test_epoch_segments_cover_the_loop_and_end_at_save_points(pkg, epochs, save_
         assert all(last - first + 1 <= sync_every for first, last, _ in segs)
     # without saving or syncing the whole run is one segment
     assert pkg.optim.epoch_segments(epochs, save_every, 0, saving=False) == [(0, epochs - 1, False)]
+
+
+def test_save_loss_writes_the_reference_plot(pkg, tmp_path):
+    """src/optimization.py:35-43: an 8 x 4 inch log(1 + loss) plot; drawn
+    without touching pyplot's global state."""
+    from PIL import Image
+    import matplotlib
+    backend = matplotlib.get_backend()
+    pkg.SvbrdfOptim.save_loss([[0.5, 0.2, 0.1], [0.05, 0.04, 0.03]], ["image loss", "feature loss"],
+                              tmp_path / "loss.jpg", 1000)
+    with Image.open(tmp_path / "loss.jpg") as im:
+        assert im.size == (800, 400)
+    assert matplotlib.get_backend() == backend
