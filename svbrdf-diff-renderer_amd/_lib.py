@@ -12,6 +12,9 @@ import torch
 _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libsvbrdf_hip.so"
 
+# the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
+ABI_VERSION = 3
+
 OK = 0
 EINVAL = -1
 ESHAPE = -2
@@ -91,6 +94,13 @@ def load():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python __graft_entry__.py build` "
                           "(or svbrdf-diff-renderer_amd/csrc/build.py)")
     lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    lib.svbrdf_abi_version.restype = ctypes.c_int
+    version = lib.svbrdf_abi_version()
+    if version != ABI_VERSION:
+        # argument lists differ between versions: calling through the wrong
+        # signatures would pass garbage pointers to the kernels
+        raise ImportError(f"{LIB_PATH} implements ABI {version}, these bindings need {ABI_VERSION}: rebuild it "
+                          "with `python __graft_entry__.py build`")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
