@@ -237,7 +237,8 @@ def torch_gpu_baseline(prob, res, views, dev, steps=5, warmup=2, sync=True):
     ms = time_events(it, steps)
     return {"value": 1000.0 / ms, "unit": "it/s", "ms_per_step": ms, "steps": steps,
             "what": "reference op sequence (src/microfacet.py:84-120) + log-L2 + torch.optim.Adam, eager, "
-                    "autograd, per-iteration loss.item() as src/svbrdf.py:64"}
+                    "autograd" + (", per-iteration loss.item() as src/svbrdf.py:64" if sync else
+                                  ", no per-iteration host read")}
 
 
 def batch_materials(pkg, dev, seed=3):
@@ -304,6 +305,63 @@ def batch_leg(pkg, dev, steps=20, warmup=3):
             "per_material_loop": {"ms_per_step": round(ms_loop, 4), "what": "FusedStep per material, 64 steps "
                                   "(one launch each) in one HIP graph"},
             "speedup_vs_loop": round(ms_loop / ms, 3)}
+
+
+def copy_bandwidth(dev, mib=1024, n=10):
+    """Attainable HBM bandwidth: a plain device copy of `mib` MiB (read and
+    written once), HIP events around n copies."""
+    src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ms = time_events(lambda: dst.copy_(src), n)
+    gbs = 2 * src.numel() * 4 / (ms * 1e-3) / 1e9
+    return {"what": f"torch copy of {mib} MiB (read + write)", "ms": round(ms, 4), "achieved_GBs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def staged_leg(pkg, dev, views, stages=(256, 512, 1024), iters=20):
+    """Config 2: 20 fused iterations at each of 256^2, 512^2 and 1024^2, each
+    stage starting from the previous one's maps upsampled x2 (bilinear,
+    align_corners=False); one graph of `iters` steps per stage.  Per-stage
+    ms and the whole run, with the final loss."""
+    syn = pkg.synthetic
+    cam, light = syn.flash_grid(n_views=views)
+    P = syn.light_power()
+    tex = None
+    out = {"stages": {}}
+    total = 0.0
+    for res in stages:
+        cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+        r = pkg.Microfacet(res, views, syn.PLANE_SIZE_CM, cl, dev)
+        gt = torch.tensor(syn.ground_truth_textures(res, seed=0), device=dev)[None]
+        with torch.no_grad():
+            target = r.eval(gt)
+        if tex is None:
+            tex = torch.tensor(syn.init_textures(gt[0].cpu().numpy(), seed=1), device=dev)[None].contiguous()
+        else:
+            tex = torch.nn.functional.interpolate(tex, scale_factor=2, mode="bilinear",
+                                                  align_corners=False).contiguous()
+        fs = pkg.FusedStep(r, tex, target, 0.01, loss="log_l2", history=iters, own_textures=True)
+        g = fs._capture_graph(iters)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        fs.steps_launched = iters
+        fs.write_back()
+        losses = fs.losses()
+        out["stages"][f"{res}^2"] = {"ms": round(ms, 4), "it_per_s": round(iters * 1000.0 / ms, 1),
+                                     "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+        total += ms
+    out["total_ms"] = round(total, 4)
+    out["what"] = (f"{iters} fused log-L2 iterations per stage at {', '.join(f'{r}^2' for r in stages)} x {views} "
+                   "views, bilinear x2 between stages, one graph replay per stage (first replay: includes "
+                   "the graph launch)")
+    return out
 
 
 def light_leg(pkg, dev, res, views, per=100, n=20):
@@ -506,6 +564,30 @@ def compose_leg(pkg, dev, res=256, views=9, iters=100):
             "loss_first": first, "loss_last": last}
 
 
+def cpu_forward(res, views, n=3):
+    """One forward render of the eager restatement on the host cores at the
+    bench shape (config 1's CPU counterpart): median of n."""
+    from oracle import microfacet_torch as MT   # the CPU baseline leg only
+    pkg = __graft_entry__.load_package()
+    syn = pkg.synthetic
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    torch.set_num_threads(threads)
+    cam, light = syn.flash_grid(n_views=views)
+    cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(syn.light_power())]
+    r = MT.MicrofacetEager(res, views, syn.PLANE_SIZE_CM, cl, "cpu")
+    gt = torch.tensor(syn.ground_truth_textures(res, seed=0))[None]
+    times = []
+    with torch.no_grad():
+        for _ in range(n):
+            t0 = time.perf_counter()
+            r.eval(gt)
+            times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 3), "unit": "renders/s", "cores": threads, "kind": "port",
+            "sample": f"{n} forward renders of the eager restatement at {res}^2 x {views} views, median s = "
+                      f"{med:.3f}"}
+
+
 def cpu_baseline(res, views, seconds):
     """The oracle's eager restatement on the host cores: the reference's CPU
     path (src/scripts.py:68 falls back to device 'cpu')."""
@@ -673,6 +755,8 @@ def main():
                       "frac": round(bb / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     out["optim_light"] = light_leg(pkg, dev, res, views)
+    out["hbm_copy"] = copy_bandwidth(dev)
+    out["staged"] = staged_leg(pkg, dev, views)
     if not args.no_batch:
         out["batched"] = batch_leg(pkg, dev)
     if not args.no_compose:
@@ -688,6 +772,7 @@ def main():
             tb = torch_gpu_baseline(prob, res, views, dev)
             out["torch_gpu_baseline"] = tb
             out["speedup_vs_torch_gpu"] = round(value / tb["value"], 1)
+            out["torch_gpu_baseline_nosync"] = torch_gpu_baseline(prob, res, views, dev, sync=False)
         except torch.cuda.OutOfMemoryError as e:     # pragma: no cover
             out["torch_gpu_baseline"] = {"error": str(e)[:200]}
     if not args.no_cpu:
@@ -695,6 +780,7 @@ def main():
         out["cpu_baseline"] = cb
         # config 0 (256^2 x 9, the reference's CPU-runnable case) as well
         out["cpu_baseline_256"] = cpu_baseline(256, views, min(args.cpu_seconds, 5.0))
+        out["cpu_forward"] = cpu_forward(res, views)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
