@@ -78,13 +78,9 @@ def _png_chunk(kind, data):
     return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
 
 
-def encode_png(arr):
-    """PNG bytes of [H, W] or [H, W, 1..4] uint8.  Every scanline uses the Sub
-    filter (each byte minus the one a pixel to its left, mod 256: one numpy
-    subtraction for the whole image), and the filtered rows are deflated in
-    independent pieces on a thread pool (zlib releases the GIL), joined as one
-    zlib stream: full flushes between the pieces, one header, and the Adler-32
-    of the whole stream -- the parallel-deflate layout any inflater reads."""
+def _png_pieces(arr):
+    """Sub-filtered scanlines of [H, W] or [H, W, 1..4] uint8, cut into deflate
+    pieces of whole rows: (IHDR payload, filtered bytes, pieces)."""
     arr = np.ascontiguousarray(arr, dtype=np.uint8)
     if arr.ndim == 2:
         arr = arr[:, :, None]
@@ -99,30 +95,88 @@ def encode_png(arr):
     data = memoryview(raw.reshape(-1))
     step = max(1, _CHUNK_BYTES // raw.shape[1]) * raw.shape[1]
     pieces = [data[i:i + step] for i in range(0, len(data), step)]
+    return struct.pack(">IIBBBBB", w, h, 8, _PNG_COLOR_TYPE[c], 0, 0, 0), data, pieces
 
-    def deflate(k):
-        z = zlib.compressobj(PNG_COMPRESS_LEVEL, zlib.DEFLATED, -15)
-        out = z.compress(pieces[k])
-        return out + z.flush(zlib.Z_FINISH if k == len(pieces) - 1 else zlib.Z_FULL_FLUSH)
-    if len(pieces) == 1:
-        body = [deflate(0)]
-    else:
-        body = list(_pool().map(deflate, range(len(pieces))))
-    stream = b"\x78\x01" + b"".join(body) + struct.pack(">I", zlib.adler32(data) & 0xFFFFFFFF)
-    ihdr = struct.pack(">IIBBBBB", w, h, 8, _PNG_COLOR_TYPE[c], 0, 0, 0)
-    return b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", ihdr) + _png_chunk(b"IDAT", stream) + _png_chunk(b"IEND", b"")
+
+def _deflate(piece, last):
+    z = zlib.compressobj(PNG_COMPRESS_LEVEL, zlib.DEFLATED, -15)
+    return z.compress(piece) + z.flush(zlib.Z_FINISH if last else zlib.Z_FULL_FLUSH)
+
+
+def _png_parts(ihdr, data, body):
+    """The file as a list of byte strings: the deflated pieces are not copied
+    into one buffer, and the IDAT checksum runs over them in turn."""
+    zhead = b"\x78\x01"
+    adler = struct.pack(">I", zlib.adler32(data) & 0xFFFFFFFF)
+    crc = zlib.crc32(zhead, zlib.crc32(b"IDAT"))
+    for piece in body:
+        crc = zlib.crc32(piece, crc)
+    crc = zlib.crc32(adler, crc)
+    size = len(zhead) + sum(len(piece) for piece in body) + len(adler)
+    return [b"\x89PNG\r\n\x1a\n", _png_chunk(b"IHDR", ihdr), struct.pack(">I", size), b"IDAT", zhead, *body,
+            adler, struct.pack(">I", crc & 0xFFFFFFFF), _png_chunk(b"IEND", b"")]
+
+
+def _pmap(fn, items):
+    items = list(items)
+    return list(_pool().map(fn, items)) if len(items) > 1 else [fn(x) for x in items]
+
+
+def _encode_parts(arrays):
+    arrays = list(arrays)
+    prepared = _pmap(_png_pieces, arrays)
+    jobs = [(i, k) for i, (_, _, pieces) in enumerate(prepared) for k in range(len(pieces))]
+
+    def run(job):
+        i, k = job
+        pieces = prepared[i][2]
+        return _deflate(pieces[k], k == len(pieces) - 1)
+    bodies = [[] for _ in arrays]
+    for (i, _), out in zip(jobs, _pmap(run, jobs)):
+        bodies[i].append(out)
+    return _pmap(lambda i: _png_parts(prepared[i][0], prepared[i][1], bodies[i]), range(len(arrays)))
+
+
+def encode_pngs(arrays):
+    """PNG bytes of several [H, W] or [H, W, 1..4] uint8 images.  Every
+    scanline uses the Sub filter (each byte minus the one a pixel to its left,
+    mod 256: one numpy subtraction per image), and the filtered rows of all
+    images are deflated in independent pieces on one thread pool (zlib and
+    numpy release the GIL), each image's pieces joined as one zlib stream:
+    full flushes between them, one header, and the Adler-32 of the whole
+    stream -- the parallel-deflate layout any inflater reads."""
+    return [b"".join(parts) for parts in _encode_parts(arrays)]
+
+
+def encode_png(arr):
+    """PNG bytes of one [H, W] or [H, W, 1..4] uint8 image (see encode_pngs)."""
+    return encode_pngs([arr])[0]
 
 
 def write_png(filename, arr):
     """Writes [H, W] or [H, W, 3 | 4] uint8 (RGB order) as an 8-bit PNG."""
-    pathlib.Path(filename).write_bytes(encode_png(arr))
+    write_pngs([(filename, arr)])
 
 
 def write_pngs(items):
-    """Writes several (filename, array) PNGs; each one's deflate runs on the
-    thread pool.  Returns when all are on disk."""
-    for f, a in items:
-        write_png(f, a)
+    """Writes several (filename, array) PNGs, all encoded and written together
+    on the thread pool.  Returns when all are on disk."""
+    items = list(items)
+    parts = _encode_parts(a for _, a in items)
+
+    def write(i):
+        with open(items[i][0], "wb") as f:
+            f.writelines(parts[i])
+    _pmap(write, range(len(items)))
+
+
+def read_pngs(filenames):
+    """read_png of several files on the thread pool (Pillow's decoder releases
+    the GIL), in order."""
+    filenames = list(filenames)
+    if len(filenames) < 2:
+        return [read_png(f) for f in filenames]
+    return list(_pool().map(read_png, filenames))
 
 
 def _device(device):

@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .imageio import (decode, encode, img9to1, imresize, montage9, montage_tex, read_png, to_device_u8,
+from .imageio import (decode, encode, img9to1, imresize, montage9, montage_tex, read_pngs, to_device_u8,
                       write_pngs)
 
 MAP_FILES = {"normal": "nom.png", "diffuse": "dif.png", "specular": "spe.png", "roughness": "rgh.png"}
@@ -139,7 +139,9 @@ class SvbrdfIO:
         d = pathlib.Path(textures_dir)
         if not d.exists():
             raise FileNotFoundError(f"{d} does not exist")
-        maps = {k: to_device_u8(read_png(d / f), self.device) for k, f in MAP_FILES.items()}
+        names = list(MAP_FILES)
+        arrays = read_pngs(d / MAP_FILES[k] for k in names)
+        maps = {k: to_device_u8(a, self.device) for k, a in zip(names, arrays)}
         return textures_from_maps_u8(maps["diffuse"], maps["normal"], maps["specular"], maps["roughness"], res)
 
     def save_textures_th(self, textures_th, textures_dir):
@@ -158,8 +160,7 @@ class SvbrdfIO:
         if not d.exists():
             raise FileNotFoundError(f"{d} does not exist")
         ims = []
-        for i in self.idx:
-            u = read_png(d / f"{i:02d}.png")
+        for i, u in zip(self.idx, read_pngs(d / f"{i:02d}.png" for i in self.idx)):
             if u.ndim != 3 or u.shape[2] != 3:
                 raise ValueError(f"{d / f'{i:02d}.png'} should be a 3 channel image")
             ims.append(_sized(to_device_u8(u, self.device), res))
