@@ -641,6 +641,8 @@ def main():
     ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (config 4)")
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
+    ap.add_argument("--no-light", action="store_true", help="skip the light-power optimisation leg")
+    ap.add_argument("--no-staged", action="store_true", help="skip the config-2 staged leg")
     ap.add_argument("--no-driver", action="store_true",
                     help="skip the end-to-end optim_perpixel leg (files in a temporary directory)")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
@@ -754,9 +756,11 @@ def main():
                       "achieved_GBs": round(bb / (bms * 1e-3) / 1e9, 1),
                       "frac": round(bb / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
-    out["optim_light"] = light_leg(pkg, dev, res, views)
+    if not args.no_light:
+        out["optim_light"] = light_leg(pkg, dev, res, views)
     out["hbm_copy"] = copy_bandwidth(dev)
-    out["staged"] = staged_leg(pkg, dev, views)
+    if not args.no_staged:
+        out["staged"] = staged_leg(pkg, dev, views)
     if not args.no_batch:
         out["batched"] = batch_leg(pkg, dev)
     if not args.no_compose:
