@@ -290,12 +290,20 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
             PF[c] = P[c] * m.Fc[c];
         }
     }
+    // dL/dimages of view k + 1 are loaded while view k is shaded
+    auto load_gout = [&](int k, float (&g)[3]) {
+        const Tensor G = tensor(gout + (size_t)(3 * k) * npix, 3, npix);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[c] = px.valid ? ld(G, c, px.off) : 0.0f;
+    };
+    float gnext[3];
+    load_gout(0, gnext);
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
-        const Tensor G = tensor(gout + (size_t)(3 * k) * npix, 3, npix);
         float go[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) go[c] = px.valid ? ld(G, c, px.off) : 0.0f;
+        for (int c = 0; c < 3; ++c) go[c] = gnext[c];
+        if (k + 1 < n_views) load_gout(k + 1, gnext);
         // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
         auto dimg = [&](float img, float g) {
             const float ic = tclamp(img, kEps, 1.0f);
@@ -350,7 +358,8 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
 }
 
 #ifndef SVBRDF_BACKWARD_WAVES
-#define SVBRDF_BACKWARD_WAVES 5      // minimum waves per SIMD for the adjoint (95 VGPRs, no spills)
+#define SVBRDF_BACKWARD_WAVES 5      // minimum waves per SIMD for the adjoint (93-96 VGPRs; the light-power
+                                     // variants spill up to 12 B, the others none)
 #endif
 template <bool LIGHT_GRAD, int TERMS = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_BACKWARD_WAVES))) void backward_kernel(
