@@ -88,11 +88,21 @@ __device__ __forceinline__ ViewPos load_view(const float *__restrict__ cam, cons
 
 // True when every view's light sits exactly at its camera (the flash setup of
 // the reference's captures): then the shading uses the co-located forms.
+// Each lane compares one coordinate and a wave vote combines them: one round
+// trip of vector loads per 64 coordinates at the start of a block (a loop of
+// scalar loads that stops at the first difference waits on 3 V dependent
+// round trips there).  The whole wave must be active: every caller reaches it
+// with all 64 lanes on (full 256-thread blocks, outside any divergent branch).
 __device__ __forceinline__ bool all_colocated(const float *__restrict__ cam, const float *__restrict__ light,
                                               int n_views) {
-    bool all = true;
-    for (int e = 0; e < 3 * n_views; ++e) all = all && (ldu(cam, e) == ldu(light, e));
-    return all;
+    const int n = 3 * n_views, lane = (int)(threadIdx.x & 63u);
+    bool differ = false;
+    for (int e0 = 0; e0 < n; e0 += 64) {
+        const int e = e0 + lane;
+        // a NaN coordinate compares unequal: not co-located
+        if (e < n) differ = differ || !(cam[e] == light[e]);
+    }
+    return __builtin_amdgcn_ballot_w64(differ) == 0;
 }
 
 // The pixel a thread owns.  Threads past the end keep working on the last

@@ -84,6 +84,24 @@ def test_forward_and_adjoint_synthetic(pkg, res, V, coloc):
     assert np.linalg.norm(dP - p64) <= max(1e-4 * np.linalg.norm(p64), 2 * np.linalg.norm(p32 - p64))
 
 
+@pytest.mark.parametrize("e", [0, 63, 64, 89])
+def test_one_offset_light_coordinate_takes_the_general_path(pkg, e):
+    """Every light at its camera except coordinate e of the 3 V flattened
+    positions (V = 30, so the co-location test spans two 64-coordinate
+    chunks): the kernels must shade with the general geometry.  The
+    co-located forms would light view e // 3 from its camera, 1.5 cm away."""
+    res, V = 12, 30
+    tex, res, size, cam, light, P = scene(pkg, res, V, seed=7)
+    light = light.copy()
+    light.reshape(-1)[e] += 1.5
+    gout = np.random.default_rng(e).standard_normal((V, 3, res, res)).astype(np.float32)
+    out, dtex, dP = gpu_render(pkg, tex, res, size, cam, light, P, gout=gout)
+    o32, g32, _ = eager_vjp(tex, res, size, cam, light, P, gout, torch.float32)
+    o64, g64, _ = eager_vjp(tex, res, size, cam, light, P, gout, torch.float64)
+    parity.check_forward(out, o32, o64, M.fp32_noise_scale(tex, res, size, cam, light, P), f"offset e={e}")
+    parity.check_gradient(dtex, g32, g64, f"offset e={e}")
+
+
 def test_forward_1024_band_and_properties(pkg):
     """Config-1 shape (1024^2 x 9): elementwise parity on a band of rows,
     range and mirror symmetry on the whole image."""

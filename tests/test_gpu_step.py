@@ -57,6 +57,26 @@ def test_first_step_gradient_and_loss(pkg, loss, mode, res, V, coloc):
     assert np.mean(np.abs(tex[0].cpu().numpy() - p_exp) <= 1e-5) >= 0.999
 
 
+@pytest.mark.parametrize("e", [0, 89])
+def test_one_offset_light_coordinate_step(pkg, e):
+    """The fused step's co-location test over two 64-coordinate chunks
+    (V = 30): one light coordinate off its camera selects the general path."""
+    res, V = 12, 30
+    gt, _, size, cam, light, P = scene(pkg, res, V, seed=5)
+    light = light.copy()
+    light.reshape(-1)[e] += 1.5
+    init = pkg.synthetic.init_textures(gt, seed=1)
+    target = M.render(gt, res, size, cam, light, P, np.float32)
+    fs, _ = make_problem(pkg, init, target, res, size, cam, light, P, "log_l2")
+    fs.step()
+    torch.cuda.synchronize()
+    g_gpu = fs.exp_avg[0].cpu().numpy().astype(np.float64) / np.float64(np.float32(0.1))
+    _, g32 = eager_loss_grad(init, target, res, size, cam, light, P, M.LOSS_LOG_L2, torch.float32)
+    L64, g64 = eager_loss_grad(init, target, res, size, cam, light, P, M.LOSS_LOG_L2, torch.float64)
+    parity.check_gradient(g_gpu, g32, g64, f"offset step e={e}")
+    assert abs(fs.loss() - L64) <= 1e-5 * abs(L64)
+
+
 @pytest.mark.parametrize("loss", ["l2", "logl2"])
 @pytest.mark.parametrize("light", [False, True])
 def test_matches_reference_loop_golden(pkg, golden_dir, loss, light):
