@@ -82,6 +82,18 @@ def algorithmic_bytes_render(res, views):
     return res * res * (36 + 12 * views)
 
 
+INFINITY_CACHE_BYTES = 256 * 2**20   # MI355X last-level (memory-side) cache
+
+
+def working_set_bytes(fs, views):
+    """Device bytes a FusedStep reads and writes every iteration: its maps
+    (at texture_stride), two Adam moments and the prepared targets (at
+    plane_stride)."""
+    npix = fs.res * fs.res
+    tps, ps = fs.texture_stride or npix, fs.plane_stride or npix   # 0: dense planes
+    return 4 * (9 * tps + 18 * ps + 3 * views * ps)
+
+
 def algorithmic_bytes_adjoint(res, views):
     # textures and dL/dimages read once, dL/dtextures written once
     return res * res * (36 + 12 * views + 36)
@@ -720,7 +732,13 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((res, views)),
                      "algorithmic_bytes": nbytes, "kernel": "fused_step_kernel",
                      "kernel_ms": round(kernel_ms, 5),
-                     "step_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "step_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     # bytes the step keeps between iterations (maps, moments, prepared
+                     # targets at their plane strides): at or below the 256 MB
+                     # Infinity Cache, part of each iteration's reads are served
+                     # from it, and `traffic` (L2-to-fabric bytes) cannot tell them apart
+                     "working_set_bytes": working_set_bytes(fs, views),
+                     "infinity_cache_bytes": INFINITY_CACHE_BYTES},
         "loss_after": loss_after,
         "per_replay_us_per_step": spread,
     }
