@@ -25,14 +25,14 @@ Printed JSON line (one):
   torch_gpu_baseline  the reference's eager op sequence + torch.optim.Adam on
                     the same GPU (it/s), and the speedup over it
   cpu_baseline      the same eager restatement on the host CPU cores (oracle/);
-                    cpu_baseline_256 the same at 256^2 (config 0)
-  forward           forward-render throughput at the same shape (config 1)
+                    cpu_baseline_256 the same at 256^2 x 9
+  forward           forward-render throughput at the same shape
   adjoint           the render adjoint alone (dL/dimages -> dL/dtextures, 180 B/pixel)
   optim_light       the fused step with the light power optimised as well
 
-  batched           config 3: 64 materials of uneven size in one persistent
+  batched           64 materials of uneven size in one persistent
                     launch (autotuned), against a per-material loop
-  composed          config 4: render adjoint under a VGG-shaped feature loss
+  composed          render adjoint under a VGG-shaped feature loss (256^2 x 9)
   codec             the map / target codec kernels (8-bit maps and images <->
                     tensors, Lanczos-4 resize, stage transition) at the same shape
   materialgan       the MaterialGAN caller: generator -> render -> feature loss ->
@@ -254,7 +254,7 @@ def torch_gpu_baseline(prob, res, views, dev, steps=5, warmup=2, sync=True):
 
 
 def batch_materials(pkg, dev, seed=3):
-    """Config 3: 64 materials (24 at 256^2, 24 at 512^2, 16 at 1024^2), views
+    """64 materials (24 at 256^2, 24 at 512^2, 16 at 1024^2), views
     drawn uniformly from 4..16, flash positions jittered +-3 cm around the
     3x3 grid; maps seeded per material, targets rendered from the truth."""
     syn = pkg.synthetic
@@ -333,7 +333,7 @@ def copy_bandwidth(dev, mib=1024, n=10):
 
 
 def staged_leg(pkg, dev, views, stages=(256, 512, 1024), iters=20):
-    """Config 2: 20 fused iterations at each of 256^2, 512^2 and 1024^2, each
+    """Coarse to fine: 20 fused iterations at each of 256^2, 512^2 and 1024^2, each
     stage starting from the previous one's maps upsampled x2 (bilinear,
     align_corners=False); one graph of `iters` steps per stage.  Per-stage
     ms and the whole run, with the final loss."""
@@ -531,7 +531,7 @@ def materialgan_leg(pkg, dev, res=256, views=9, iters=20):
 
 
 def compose_leg(pkg, dev, res=256, views=9, iters=100):
-    """Config 4: render -> VGG-19-shaped random-init conv stack -> Gram L2 on
+    """Render -> VGG-19-shaped random-init conv stack -> Gram L2 on
     5 layers + pixel L2 -> backward (HIP adjoint) -> torch Adam, 100 steps."""
     from svbrdf_amd import compose
     syn = pkg.synthetic
@@ -578,7 +578,7 @@ def compose_leg(pkg, dev, res=256, views=9, iters=100):
 
 def cpu_forward(res, views, n=3):
     """One forward render of the eager restatement on the host cores at the
-    bench shape (config 1's CPU counterpart): median of n."""
+    bench shape (the CPU counterpart of the forward line): median of n."""
     from oracle import microfacet_torch as MT   # the CPU baseline leg only
     pkg = __graft_entry__.load_package()
     syn = pkg.synthetic
@@ -649,12 +649,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-torch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-batch", action="store_true", help="skip the 64-material batched leg (config 3)")
-    ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (config 4)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-material batched leg")
+    ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (256^2 x 9)")
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
     ap.add_argument("--no-light", action="store_true", help="skip the light-power optimisation leg")
-    ap.add_argument("--no-staged", action="store_true", help="skip the config-2 staged leg")
+    ap.add_argument("--no-staged", action="store_true", help="skip the 256^2 -> 512^2 -> 1024^2 staged leg")
     ap.add_argument("--no-driver", action="store_true",
                     help="skip the end-to-end optim_perpixel leg (files in a temporary directory)")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of graph replays")
@@ -669,8 +669,8 @@ def main():
                          "calibrates the profiler's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the fused step runs on a single MI355X here. Across GPUs it would "
+              "shard by row bands of the maps, with no exchange on the gradient path.", file=sys.stderr)
         sys.exit(2)
 
     pkg = __graft_entry__.load_package()
@@ -744,13 +744,15 @@ def main():
     }
     out.update(probes)
 
-    # forward render at the same shape (config 1)
+    # forward render at the same shape (1024^2 x 9 views)
     r = prob["renderer"]
     gt = torch.tensor(prob["gt"], device=dev)[None]
+    # 50 untimed launches first: these kernels run below a second, and the
+    # clock settles over the first tens of launches
     with torch.no_grad():
-        for _ in range(5):
+        for _ in range(50):
             r.eval(gt)
-        fms = time_events(lambda: r.eval(gt), 50)
+        fms = time_events(lambda: r.eval(gt), 200)
     fb = algorithmic_bytes_render(res, views)
     out["forward"] = {"value": round(1000.0 / fms, 1), "unit": "renders/s", "ms": round(fms, 5),
                       "achieved_GBs": round(fb / (fms * 1e-3) / 1e9, 1),
@@ -766,9 +768,9 @@ def main():
     def adjoint():
         L.call("svbrdf_render_backward_terms", L.ptr(gt), res, views, r.size, L.ptr(r.camera_pos),
                L.ptr(r.light_pos), L.ptr(r.light_pow), 0, L.ptr(gout), L.ptr(gtex), None, None, 0, stream)
-    for _ in range(5):
+    for _ in range(50):
         adjoint()
-    bms = time_events(adjoint, 50)
+    bms = time_events(adjoint, 200)
     bb = algorithmic_bytes_adjoint(res, views)
     out["adjoint"] = {"value": round(1000.0 / bms, 1), "unit": "adjoints/s", "ms": round(bms, 5),
                       "achieved_GBs": round(bb / (bms * 1e-3) / 1e9, 1),
@@ -800,7 +802,7 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(res, views, args.cpu_seconds)
         out["cpu_baseline"] = cb
-        # config 0 (256^2 x 9, the reference's CPU-runnable case) as well
+        # 256^2 x 9 (the reference's CPU-runnable case) as well
         out["cpu_baseline_256"] = cpu_baseline(256, views, min(args.cpu_seconds, 5.0))
         out["cpu_forward"] = cpu_forward(res, views)
     if args.copy_calibration:

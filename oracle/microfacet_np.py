@@ -238,7 +238,7 @@ def target_floor(dtype=np.float32):
 def loss(out, target, mode, dtype=np.float32):
     """Image loss.  L2: src/optimization.py:14,29 (MSELoss, mean reduction).
 
-    LOG_L2 (not in the reference; the configs ask for it): mean of
+    LOG_L2 (not in the reference; the benchmark workloads use it): mean of
     (ln out - ln clamp(target, eps^(1/2.2), 1))^2.  The clamp keeps a
     target pixel the renderer cannot reach (e.g. a black photo pixel) finite.
     """

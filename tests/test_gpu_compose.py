@@ -1,4 +1,4 @@
-"""Config 4: the render adjoint receives dL/dimages from torch's convolution
+"""256^2 x 9 views: the render adjoint receives dL/dimages from torch's convolution
 backward (a VGG-19-shaped feature loss).  The texture gradient through the
 HIP renderer is compared with the same loss through the eager restatement of
 the reference renderer (FP32 and FP64), under tests/parity.py's criteria."""

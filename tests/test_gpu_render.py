@@ -103,7 +103,7 @@ def test_one_offset_light_coordinate_takes_the_general_path(pkg, e):
 
 
 def test_forward_1024_band_and_properties(pkg):
-    """Config-1 shape (1024^2 x 9): elementwise parity on a band of rows,
+    """Forward render at 1024^2 x 9: elementwise parity on a band of rows,
     range and mirror symmetry on the whole image."""
     res, V = 1024, 9
     tex, _, size, cam, light, P = scene(pkg, res, V, seed=0)

@@ -100,7 +100,7 @@ def test_matches_reference_loop_golden(pkg, golden_dir, loss, light):
 
 
 def test_config0_twenty_steps_against_eager_torch_adam(pkg):
-    """Config 0: 256^2, 9 views, 20 Adam iterations at lr 0.01, log-space L2;
+    """256^2, 9 views, 20 Adam iterations at lr 0.01, log-space L2;
     the reference loop is the eager restatement driven by torch.optim.Adam."""
     res = 256
     gt, _, size, cam, light, P = scene(pkg, res, 9, seed=0)
@@ -256,7 +256,7 @@ def test_state_dict_resume_in_a_fresh_workspace(pkg):
 
 
 def test_gradient_against_fp64_finite_differences(pkg):
-    """Config 2's check: 64 random pixels x 9 parameters, FP64 central
+    """Coarse-to-fine check: 64 random pixels x 9 parameters, FP64 central
     differences (eps 1e-4) of the oracle loss; kinks are skipped."""
     res, V = 256, 9
     gt, _, size, cam, light, P = scene(pkg, res, V, seed=2)
@@ -298,7 +298,7 @@ def test_gradient_against_fp64_finite_differences(pkg):
 
 
 def test_staged_256_512_1024(pkg):
-    """Config 2: 256^2 -> 512^2 -> 1024^2, 9 views, 20 iterations per stage,
+    """Coarse to fine: 256^2 -> 512^2 -> 1024^2, 9 views, 20 iterations per stage,
     each stage starting from the bilinearly upsampled previous maps
     (F.interpolate(mode='bilinear', align_corners=False)); the material is
     one 1024^2 ground truth, box-averaged for the coarse stages.  The loss
