@@ -29,6 +29,8 @@ Printed JSON line (one):
   forward           forward-render throughput at the same shape
   adjoint           the render adjoint alone (dL/dimages -> dL/dtextures, 180 B/pixel)
   optim_light       the fused step with the light power optimised as well
+  cache_cold        the fused step with its working set kept out of the
+                    256 MB Infinity Cache (two problems stepping alternately)
 
   batched           64 materials of uneven size in one persistent
                     launch (autotuned), against a per-material loop
@@ -46,6 +48,7 @@ Printed JSON line (one):
                     [--cpu-seconds 20] [--graph-steps 100 | --eager] [--no-eager-probes]
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -394,6 +397,33 @@ def light_leg(pkg, dev, res, views, per=100, n=20):
             "light_pow_after": [round(float(x), 3) for x in fs.light_pow.cpu()]}
 
 
+def cache_cold_leg(pkg, dev, res, views, per=50, n=10):
+    """The fused step of the bench shape with its working set kept out of
+    the Infinity Cache: two independent problems (2 x 226.6 MB at 1024^2 x 9)
+    step alternately inside one graph, so each step finds the other one's
+    data in the cache rather than its own.  Median us per step over n
+    replays of 2 x `per` steps."""
+    steps = [make_problem(pkg, res, views, dev)["fused"] for _ in range(2)]
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream(device=dev)
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        g.capture_begin()
+        for _ in range(per):
+            for fs in steps:
+                fs._launch(ctypes.c_void_p(s.cuda_stream))
+        g.capture_end()
+    torch.cuda.current_stream().wait_stream(s)
+    g.replay()
+    sp = replay_spread(lambda k: g.replay(), 2 * per, n)
+    nbytes = algorithmic_bytes_step(res, views)
+    return {"us_per_step": sp["median"], "iqr": sp["iqr"], "value": round(1e6 / sp["median"], 1), "unit": "it/s",
+            "achieved_GBs": round(nbytes / (sp["median"] * 1e-6) / 1e9, 1),
+            "frac": round(nbytes / (sp["median"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "working_set_bytes": sum(working_set_bytes(fs, views) for fs in steps),
+            "loss_after": [fs.loss() for fs in steps]}
+
+
 def driver_leg(pkg, dev, res=1024, epochs=1000):
     """The reference's per-pixel workflow end to end (src/scripts.py:31-41,
     67-97; run.py:37's 1024^2 / lr 0.01 / 1000 epochs from random maps): a
@@ -654,6 +684,8 @@ def main():
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
     ap.add_argument("--no-light", action="store_true", help="skip the light-power optimisation leg")
+    ap.add_argument("--no-cache-cold", action="store_true",
+                    help="skip the leg that keeps the step's working set out of the Infinity Cache")
     ap.add_argument("--no-staged", action="store_true", help="skip the 256^2 -> 512^2 -> 1024^2 staged leg")
     ap.add_argument("--no-driver", action="store_true",
                     help="skip the end-to-end optim_perpixel leg (files in a temporary directory)")
@@ -778,6 +810,8 @@ def main():
 
     if not args.no_light:
         out["optim_light"] = light_leg(pkg, dev, res, views)
+    if not args.no_cache_cold:
+        out["cache_cold"] = cache_cold_leg(pkg, dev, res, views)
     out["hbm_copy"] = copy_bandwidth(dev)
     if not args.no_staged:
         out["staged"] = staged_leg(pkg, dev, views)
