@@ -355,8 +355,7 @@ def staged_leg(pkg, dev, views, stages=(256, 512, 1024), iters=20):
         if tex is None:
             tex = torch.tensor(syn.init_textures(gt[0].cpu().numpy(), seed=1), device=dev)[None].contiguous()
         else:
-            tex = torch.nn.functional.interpolate(tex, scale_factor=2, mode="bilinear",
-                                                  align_corners=False).contiguous()
+            tex = pkg.upsample_textures(tex)
         fs = pkg.FusedStep(r, tex, target, 0.01, loss="log_l2", history=iters, own_textures=True)
         g = fs._capture_graph(iters)
         torch.cuda.synchronize()

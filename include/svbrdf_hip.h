@@ -202,6 +202,18 @@ int svbrdf_fused_step_flush(void *state, int res, int n_views, float *loss_histo
 int svbrdf_resize_lanczos4_u8(const unsigned char *src, int src_h, int src_w, int channels,
                               unsigned char *dst, int dst_h, int dst_w, void *stream);
 
+/* Coarse-to-fine stage chaining in memory: the reference chains stages
+ * through 8-bit PNG files and a Lanczos-4 resize (src/scripts.py:83-85,
+ * src/svbrdf.py:150-166, src/imageio.py:75-76; svbrdf_resize_lanczos4_u8
+ * above).  This is the float alternative with the semantics of
+ * torch.nn.functional.interpolate(scale_factor=2, mode="bilinear",
+ * align_corners=False): n_planes planes of res x res floats, src_stride
+ * floats apart (0: res^2), to planes of 2res x 2res, dst_stride floats apart
+ * (0: 4 res^2; the next stage's svbrdf_plane_stride(2 res) layout fits).
+ * res up to 8192, n_planes up to 64; the buffers must not overlap. */
+int svbrdf_upsample2x_bilinear(const float *src, int res, size_t src_stride, float *dst, size_t dst_stride,
+                               int n_planes, void *stream);
+
 /* imread flags (src/imageio.py:26-47). */
 enum svbrdf_imread_flag {
     SVBRDF_IMREAD_NONE = 0,    /* u / 255 per channel                               */

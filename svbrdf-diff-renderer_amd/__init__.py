@@ -24,9 +24,9 @@ All compute runs in libsvbrdf_hip.so (gfx950); nothing falls back to the CPU.
 from . import _lib, imageio, materialgan, scripts, synthetic
 from ._lib import LOSS_L2, LOSS_LOG_L2, SvbrdfHipError
 from .batch import BatchedStep
-from .optim import FusedStep, SvbrdfOptim, loss_mode
+from .optim import FusedStep, SvbrdfOptim, loss_mode, upsample_textures
 from .renderer import Microfacet, RenderFunction
 from .svbrdf_io import SvbrdfIO, reload_textures
 
 __all__ = ["Microfacet", "RenderFunction", "SvbrdfOptim", "FusedStep", "BatchedStep", "SvbrdfIO", "reload_textures",
-           "imageio", "materialgan", "scripts", "loss_mode", "synthetic", "LOSS_L2", "LOSS_LOG_L2", "SvbrdfHipError"]
+           "imageio", "materialgan", "scripts", "loss_mode", "upsample_textures", "synthetic", "LOSS_L2", "LOSS_LOG_L2", "SvbrdfHipError"]

@@ -58,6 +58,8 @@ SIGNATURES = {
                                                ctypes.c_size_t, _vp]),
     "svbrdf_resize_lanczos4_u8": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                                  ctypes.c_int, _vp]),
+    "svbrdf_upsample2x_bilinear": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_size_t, _vp, ctypes.c_size_t,
+                                                  ctypes.c_int, _vp]),
     "svbrdf_images_from_u8": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, _vp, _vp]),
     "svbrdf_textures_from_u8": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),

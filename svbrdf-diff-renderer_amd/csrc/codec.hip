@@ -14,6 +14,7 @@
 //   textures_from_u8_kernel 4 maps u8 -> [9, R, R] f32             10-12 B read, 36 B written / px
 //   textures_to_u8_kernel   [9, R, R] f32 -> 4 maps u8             36 B read, 10 B written / px
 //   images_to_u8_kernel     [n, C, h, w] f32 -> [n, h, w, C] u8    4 C B read, C B written / px
+//   upsample2x_kernel       [P, R, R] f32 -> [P, 2R, 2R] f32 bilinear (stage chaining in memory)
 //
 // Float arithmetic follows the reference's float32 numpy / torch op order one
 // rounding per op: contraction into fused multiply-adds is off in this file,
@@ -476,6 +477,40 @@ __global__ __launch_bounds__(kBlock) void images_to_u8_x4_kernel(const float *__
     store_px4(out, g, b);
 }
 
+// x2 bilinear upsample of n_planes planes, as torch's upsample_bilinear2d with
+// align_corners=False and scale_factor=2 computes it: source coordinate
+// max(0.5 (d + 0.5) - 0.5, 0) in float, its integer part and fraction as the
+// row / column and weight, the neighbour clamped at the last row / column,
+// and out = h0 (w0 a + w1 b) + h1 (w0 c + w1 d).  One thread per output pixel
+// walks the planes.
+__device__ __forceinline__ void up2_axis(unsigned d, unsigned n_src, unsigned &i0, unsigned &i1, float &l0,
+                                         float &l1) {
+    const float r = fmaxf(0.5f * ((float)d + 0.5f) - 0.5f, 0.0f);
+    i0 = (unsigned)r;
+    i1 = i0 + (i0 < n_src - 1u ? 1u : 0u);
+    l1 = r - (float)i0;
+    l0 = 1.0f - l1;
+}
+
+__global__ __launch_bounds__(kBlock) void upsample2x_kernel(const float *__restrict__ src, unsigned res,
+                                                            size_t src_stride, float *__restrict__ dst,
+                                                            size_t dst_stride, int n_planes) {
+    const unsigned side = 2u * res;
+    const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= (size_t)side * side) return;
+    const unsigned i = (unsigned)(p / side), j = (unsigned)(p - (size_t)i * side);
+    unsigned h0, h1, w0, w1;
+    float hl0, hl1, wl0, wl1;
+    up2_axis(i, res, h0, h1, hl0, hl1);
+    up2_axis(j, res, w0, w1, wl0, wl1);
+    const size_t r0 = (size_t)h0 * res, r1 = (size_t)h1 * res;
+    for (int c = 0; c < n_planes; ++c) {
+        const float *s = src + (size_t)c * src_stride;
+        const float a = s[r0 + w0], b = s[r0 + w1], e = s[r1 + w0], f = s[r1 + w1];
+        dst[(size_t)c * dst_stride + p] = hl0 * (wl0 * a + wl1 * b) + hl1 * (wl0 * e + wl1 * f);
+    }
+}
+
 bool aligned(const void *p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
 
 unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -522,6 +557,23 @@ int svbrdf_resize_lanczos4_u8(const unsigned char *src, int src_h, int src_w, in
         case 3: hipLaunchKernelGGL(resize_lanczos4_kernel<3>, grid, dim3(kBlock), 0, s, src, src_h, src_w, dst, dst_h, dst_w); break;
         default: hipLaunchKernelGGL(resize_lanczos4_kernel<4>, grid, dim3(kBlock), 0, s, src, src_h, src_w, dst, dst_h, dst_w); break;
     }
+    return check_launch(what);
+}
+
+int svbrdf_upsample2x_bilinear(const float *src, int res, size_t src_stride, float *dst, size_t dst_stride,
+                               int n_planes, void *stream) {
+    const char *what = "svbrdf_upsample2x_bilinear";
+    if (!src || !dst) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (res < 1 || res > kMaxSide / 2) return fail(SVBRDF_ESHAPE, "%s: res=%d outside [1, %d]", what, res, kMaxSide / 2);
+    if (n_planes < 1 || n_planes > 64) return fail(SVBRDF_EINVAL, "%s: n_planes=%d outside [1, 64]", what, n_planes);
+    const size_t n_in = (size_t)res * res, n_out = 4 * n_in;
+    if (src_stride == 0) src_stride = n_in;
+    if (dst_stride == 0) dst_stride = n_out;
+    if (src_stride < n_in || dst_stride < n_out)
+        return fail(SVBRDF_ESHAPE, "%s: plane strides %zu / %zu below %zu / %zu", what, src_stride, dst_stride, n_in,
+                    n_out);
+    hipLaunchKernelGGL(upsample2x_kernel, dim3(blocks_for(n_out)), dim3(kBlock), 0, (hipStream_t)stream, src,
+                       (unsigned)res, src_stride, dst, dst_stride, n_planes);
     return check_launch(what);
 }
 

@@ -54,6 +54,26 @@ def _planes(buf, shape, stride):
     return buf.as_strided(shape, strides)
 
 
+def upsample_textures(textures):
+    """The maps of the next coarse-to-fine stage: ``[1, 9, R, R]`` float32
+    on the GPU -> ``[1, 9, 2R, 2R]``, with the semantics of
+    ``torch.nn.functional.interpolate(textures, scale_factor=2,
+    mode="bilinear", align_corners=False)``, computed by
+    svbrdf_upsample2x_bilinear.  The reference chains stages through 8-bit
+    PNG files and a Lanczos-4 resize instead (src/scripts.py:83-85; that
+    route is :func:`svbrdf_io.reload_textures`)."""
+    if textures.dim() != 4 or textures.shape[0] != 1 or textures.shape[2] != textures.shape[3]:
+        raise ValueError(f"textures must be [1, C, R, R], got {tuple(textures.shape)}")
+    if textures.dtype != torch.float32 or textures.device.type != "cuda":
+        raise ValueError("textures must be float32 on a GPU (there is no CPU path)")
+    src = textures.detach().contiguous()
+    n_planes, res = src.shape[1], src.shape[2]
+    out = torch.empty((1, n_planes, 2 * res, 2 * res), dtype=torch.float32, device=src.device)
+    _lib.call("svbrdf_upsample2x_bilinear", _lib.ptr(src), res, 0, _lib.ptr(out), 0, n_planes,
+              _lib.stream_handle(src.device))
+    return out
+
+
 def epoch_segments(epochs, save_every=100, sync_every=0, saving=True):
     """Splits an optimisation of `epochs` iterations at the epochs after which
     the host acts: [(first, last, save)], 0-based and inclusive.  `save` marks

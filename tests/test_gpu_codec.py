@@ -170,3 +170,55 @@ def test_codec_argument_validation(pkg):
     assert lib.svbrdf_images_from_u8(p, 1, 2, 2, 3, 9, p, None) == -1      # unknown flag
     assert lib.svbrdf_textures_from_u8(p, p, p, p, 2, 4, p, None) == -1
     assert lib.svbrdf_textures_to_u8(p, 0, p, p, p, p, None) == -2
+
+
+@pytest.mark.parametrize("res", [1, 2, 5, 64, 256])
+def test_upsample2x_matches_torch_interpolate(pkg, res):
+    """Stage chaining in memory: svbrdf_upsample2x_bilinear against
+    torch.nn.functional.interpolate(scale_factor=2, mode="bilinear",
+    align_corners=False) on the GPU and, in float64, on the CPU.  The maps
+    lie in [-1, 1]; the kernel rounds once per operation, so it stays within
+    a few float32 ulps of 1 of both."""
+    rng = np.random.default_rng(res)
+    tex = torch.tensor(rng.uniform(-1, 1, (1, 9, res, res)).astype(np.float32), device=DEV)
+    out = pkg.upsample_textures(tex)
+    ref = torch.nn.functional.interpolate(tex, scale_factor=2, mode="bilinear", align_corners=False)
+    ref64 = torch.nn.functional.interpolate(tex.cpu().double(), scale_factor=2, mode="bilinear", align_corners=False)
+    torch.cuda.synchronize()
+    assert out.shape == (1, 9, 2 * res, 2 * res)
+    assert (out - ref).abs().max().item() <= 4e-7
+    assert (out.cpu().double() - ref64).abs().max().item() <= 2.5e-7
+    # the edge rows and columns repeat the source's (torch clamps the source coordinate at 0)
+    assert torch.equal(out[0, :, 0, 0], tex[0, :, 0, 0])
+
+
+def test_upsample2x_padded_planes_and_validation(pkg):
+    """Source and destination at padded plane strides (the next stage's
+    svbrdf_plane_stride layout): the result matches the dense call and the
+    padding between planes is left as it was; bad strides and sizes raise."""
+    L = pkg._lib
+    res = 96
+    rng = np.random.default_rng(7)
+    tex = torch.tensor(rng.uniform(-1, 1, (1, 9, res, res)).astype(np.float32), device=DEV)
+    dense = pkg.upsample_textures(tex)
+    ps_src = L.query("svbrdf_plane_stride", res)
+    ps_dst = L.query("svbrdf_plane_stride", 2 * res)
+    src = torch.full((9 * ps_src,), float("nan"), device=DEV)
+    src.view(9, ps_src)[:, :res * res].copy_(tex.reshape(9, -1))
+    dst = torch.full((9 * ps_dst,), 7.0, device=DEV)
+    stream = L.stream_handle(DEV)
+    L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, ps_src, L.ptr(dst), ps_dst, 9, stream)
+    torch.cuda.synchronize()
+    planes = dst.view(9, ps_dst)
+    assert torch.equal(planes[:, :4 * res * res], dense.reshape(9, -1))
+    assert bool((planes[:, 4 * res * res:] == 7.0).all())
+    with pytest.raises(ValueError):
+        L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, res * res - 1, L.ptr(dst), 0, 9, stream)
+    with pytest.raises(ValueError):
+        L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, 0, L.ptr(dst), 4 * res * res - 1, 9, stream)
+    with pytest.raises(ValueError):
+        L.call("svbrdf_upsample2x_bilinear", L.ptr(src), 0, 0, L.ptr(dst), 0, 9, stream)
+    with pytest.raises(ValueError):
+        L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, 0, L.ptr(dst), 0, 65, stream)
+    with pytest.raises(ValueError):
+        pkg.upsample_textures(tex.cpu())

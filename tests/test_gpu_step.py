@@ -300,7 +300,8 @@ def test_gradient_against_fp64_finite_differences(pkg):
 def test_staged_256_512_1024(pkg):
     """Coarse to fine: 256^2 -> 512^2 -> 1024^2, 9 views, 20 iterations per stage,
     each stage starting from the bilinearly upsampled previous maps
-    (F.interpolate(mode='bilinear', align_corners=False)); the material is
+    (svbrdf_upsample2x_bilinear, F.interpolate(mode='bilinear',
+    align_corners=False) semantics); the material is
     one 1024^2 ground truth, box-averaged for the coarse stages.  The loss
     falls in every stage and every stage ends closer to its ground truth
     than it started."""
@@ -318,8 +319,7 @@ def test_staged_256_512_1024(pkg):
         if tex is None:
             tex = torch.tensor(pkg.synthetic.init_textures(gt, seed=1), device=DEV)[None].contiguous()
         else:
-            tex = torch.nn.functional.interpolate(tex, size=(res, res), mode="bilinear",
-                                                  align_corners=False).contiguous()
+            tex = pkg.upsample_textures(tex)
         start_err = (tex[0] - torch.tensor(gt, device=DEV)).abs().mean().item()
         fs = pkg.FusedStep(r, tex, target, 0.01, loss="log_l2", history=20)
         for _ in range(20):
