@@ -506,6 +506,9 @@ def codec_leg(pkg, dev, res=1024, views=9, iters=20):
             lambda: io.imresize(small[0], (res, res)), (npx // 4) * 3 + npx * 3),
         f"stage transition {res // 2}->{res} (save + 4 resizes + load)": (
             None, lambda: sio.reload_textures(half, res), None),
+        f"stage transition in memory {res // 2}->{res} (svbrdf_upsample2x_bilinear)": (
+            lambda: L.call("svbrdf_upsample2x_bilinear", L.ptr(half), res // 2, 0, L.ptr(tex_out), 0, 9, st),
+            lambda: pkg.upsample_textures(half), 9 * 4 * (npx // 4 + npx)),
     }
     out = {}
     for name, (kernel, api, nbytes) in cases.items():
