@@ -189,7 +189,7 @@ int svbrdf_fused_step_flush(void *state, int res, int n_views, float *loss_histo
 
 /* ------------------------------------------------------------------------
  * Map / target codec: the per-pixel work between PNG files and the tensors
- * above (src/imageio.py:11-76, src/svbrdf.py:126-132,168-237).  The PNG
+ * above (src/imageio.py:11-76, src/svbrdf.py:126-132,150-221).  The PNG
  * container (inflate / deflate, scanline filters) stays on the host; these
  * take and produce 8-bit images in device memory, RGB-ordered and
  * channel-interleaved ([H, W, C] bytes), as the files store them.
@@ -222,14 +222,14 @@ enum svbrdf_imread_flag {
     SVBRDF_IMREAD_NORMAL = 3,  /* 3 channels, u / 255 * 2 - 1 divided by its 2-norm  */
 };
 
-/* imread after the resize (src/imageio.py:11-49; SvbrdfIO.load_images_th,
- * src/svbrdf.py:209-221, is n "srgb" images): n images [n, height, width,
+/* imread after the resize (src/imageio.py:11-51; SvbrdfIO.load_images_th,
+ * src/svbrdf.py:191-204, is n "srgb" images): n images [n, height, width,
  * channels] bytes -> [n, C', height, width] float32, C' = channels, 1 for
  * SVBRDF_IMREAD_ROUGH, 3 for SVBRDF_IMREAD_NORMAL. */
 int svbrdf_images_from_u8(const unsigned char *images, int n, int height, int width, int channels,
                           int flag, float *out, void *stream);
 
-/* SvbrdfIO.load_textures_th (src/svbrdf.py:168-183) after the resizes: the
+/* SvbrdfIO.load_textures_th (src/svbrdf.py:150-166) after the resizes: the
  * diffuse, normal and specular maps [res, res, 3] and the roughness map
  * [res, res, rgh_channels] (1, or 3 averaged) -> textures [9, res, res]:
  * u / 255 * 2 - 1 for diffuse / specular / roughness, the normal decoded as
@@ -239,17 +239,17 @@ int svbrdf_textures_from_u8(const unsigned char *dif, const unsigned char *nom,
                             const unsigned char *spe, const unsigned char *rgh,
                             int rgh_channels, int res, float *textures, void *stream);
 
-/* SvbrdfIO.save_textures_th (src/svbrdf.py:185-207) before the PNG encode:
+/* SvbrdfIO.save_textures_th (src/svbrdf.py:168-189) before the PNG encode:
  * textures [9, res, res] -> dif / nom / spe [res, res, 3] and rgh
  * [res, res] bytes: (t + 1) / 2 clipped to [0, 1] (normal: reconstructed
  * as SvbrdfIO.reconstruct_normal, clipped to [-1, 1], then (n + 1) / 2),
- * times 255, truncated (src/imageio.py:52-72). */
+ * times 255, truncated (src/imageio.py:54-72). */
 int svbrdf_textures_to_u8(const float *textures, int res, unsigned char *dif,
                           unsigned char *nom, unsigned char *spe, unsigned char *rgh,
                           void *stream);
 
-/* imwrite before the PNG encode (src/imageio.py:52-72; SvbrdfIO.save_images_th,
- * src/svbrdf.py:223-237, is n "srgb" images): [n, channels, height, width]
+/* imwrite before the PNG encode (src/imageio.py:54-72; SvbrdfIO.save_images_th,
+ * src/svbrdf.py:206-221, is n "srgb" images): [n, channels, height, width]
  * float32 -> [n, height, width, channels] bytes = (v * 255) truncated, with v
  * clipped to [0, 1] (SRGB, ROUGH with 1 channel, and NONE, whose unclipped
  * reference cast is only defined there) or (clip(v, -1, 1) + 1) / 2 (NORMAL). */

@@ -2,24 +2,24 @@
 TEST INFRASTRUCTURE ONLY (see oracle/__init__).
 
 What the reference does between PNG files and the tensors the renderer
-takes (src/imageio.py:11-76, src/svbrdf.py:126-132,168-221):
+takes (src/imageio.py:11-76, src/svbrdf.py:126-132,150-221):
 
-* ``imread(fn, flag, (res, res))`` (imageio.py:11-49): cv2 reads 8-bit BGR,
+* ``imread(fn, flag, (res, res))`` (imageio.py:11-51): cv2 reads 8-bit BGR,
   ``imresize`` resizes the uint8 image with ``cv2.INTER_LANCZOS4``
   (imageio.py:75-76), the result becomes float32 ``u / 255``, then per flag:
   "srgb" reorders to RGB, "rough" averages the 3 channels (or keeps 1),
   "normal" reorders to RGB, maps ``*2 - 1`` and divides by the per-pixel
   2-norm (np.linalg.norm: sqrt of the sequential sum of squares).
-* ``load_textures_th`` (svbrdf.py:168-183) maps diffuse / specular /
+* ``load_textures_th`` (svbrdf.py:150-166) maps diffuse / specular /
   roughness through ``*2 - 1`` and packs [dif 0:3, normal xy 3:5, rgh 5,
-  spe 6:9]; ``load_images_th`` (svbrdf.py:209-221) stacks "srgb" images.
-* ``save_textures_th`` (svbrdf.py:185-207) writes ``(t + 1) / 2`` for
+  spe 6:9]; ``load_images_th`` (svbrdf.py:191-204) stacks "srgb" images.
+* ``save_textures_th`` (svbrdf.py:168-189) writes ``(t + 1) / 2`` for
   diffuse / specular / roughness and ``reconstruct_normal`` (svbrdf.py:
   126-132: clamp to [-1, 1], z = sqrt(1 - clamp(x^2 + y^2, 0, 1)), divide by
-  the 2-norm) for the normal; ``imwrite`` (imageio.py:52-72) clips ("srgb",
+  the 2-norm) for the normal; ``imwrite`` (imageio.py:54-72) clips ("srgb",
   "rough": [0, 1]; "normal": [-1, 1] then ``(n + 1) / 2``) and stores
   ``(v * 255).astype(uint8)``, i.e. truncation toward zero.
-  ``save_images_th`` (svbrdf.py:223-237) writes images through "srgb".
+  ``save_images_th`` (svbrdf.py:206-221) writes images through "srgb".
 
 Arrays here are RGB-ordered (the channel reorders of the reference cancel
 between cv2's BGR and the file's RGB).  Every float op is float32 and
@@ -129,7 +129,7 @@ def u8_to_unit(u):
 
 
 def imread_u8(u, flag, dim=None):
-    """src/imageio.py:11-49 after the file decode, for an RGB-ordered uint8
+    """src/imageio.py:11-51 after the file decode, for an RGB-ordered uint8
     image; dim = (w, h) as cv2 takes it."""
     u = np.asarray(u, np.uint8)
     if dim is not None:
@@ -159,7 +159,7 @@ def imread_u8(u, flag, dim=None):
 
 
 def load_textures(dif_u8, nom_u8, spe_u8, rgh_u8, res):
-    """SvbrdfIO.load_textures_th (src/svbrdf.py:168-183) -> [9, res, res]."""
+    """SvbrdfIO.load_textures_th (src/svbrdf.py:150-166) -> [9, res, res]."""
     dim = (res, res)
     normal = imread_u8(nom_u8, "normal", dim)
     diffuse = imread_u8(dif_u8, "srgb", dim)
@@ -175,7 +175,7 @@ def load_textures(dif_u8, nom_u8, spe_u8, rgh_u8, res):
 
 
 def load_images(images_u8, res):
-    """SvbrdfIO.load_images_th (src/svbrdf.py:209-221): [n, H, W, 3] uint8 ->
+    """SvbrdfIO.load_images_th (src/svbrdf.py:191-204): [n, H, W, 3] uint8 ->
     [n, 3, res, res] float32."""
     return np.stack([imread_u8(im, "srgb", (res, res)).transpose(2, 0, 1) for im in images_u8]).astype(np.float32)
 
@@ -198,8 +198,8 @@ def reconstruct_normal_io(nxy):
 
 
 def save_textures(tex):
-    """SvbrdfIO.save_textures_th (src/svbrdf.py:185-207) + imwrite flags
-    (src/imageio.py:52-72) on [9, R, R] float32 -> (dif [R,R,3], nom [R,R,3],
+    """SvbrdfIO.save_textures_th (src/svbrdf.py:168-189) + imwrite flags
+    (src/imageio.py:54-72) on [9, R, R] float32 -> (dif [R,R,3], nom [R,R,3],
     spe [R,R,3], rgh [R,R]) uint8, RGB order."""
     tex = np.asarray(tex, np.float32)
     one, two = np.float32(1), np.float32(2)

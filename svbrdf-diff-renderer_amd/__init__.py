@@ -10,7 +10,7 @@ Drop-in for the hot path of tflsguoyu/svbrdf-diff-renderer:
   launch per iteration, with an autotuned launch shape
 * :mod:`materialgan` -- the MaterialGAN caller (src/materialgan.py) on the
   HIP render adjoint, with seeded stand-ins for the downloaded networks
-* :class:`SvbrdfIO`, :mod:`imageio` -- src/svbrdf.py:85-237 and
+* :class:`SvbrdfIO`, :mod:`imageio` -- src/svbrdf.py:86-221 and
   src/imageio.py: maps and targets between 8-bit PNG files and tensors, the
   per-pixel codec and the Lanczos-4 resize on the GPU
 * :mod:`scripts` -- the reference's driver functions (render,

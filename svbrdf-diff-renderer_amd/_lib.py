@@ -131,5 +131,18 @@ def ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def cuda_device(device, what="svbrdf_amd"):
+    """The torch.device of a GPU with its index filled in: "cuda" becomes
+    "cuda:<current device>", so that it compares equal to the device of the
+    tensors created on it.  Raises RuntimeError for a non-GPU device (there is
+    no CPU path)."""
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError(f"{what} on {dev}: the HIP library runs on a GPU only (there is no CPU path)")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
 def stream_handle(device=None):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -64,7 +64,7 @@ class BatchedStep:
                  own_textures=False):
         if not problems:
             raise ValueError("empty batch")
-        self.device = problems[0][0].device
+        self.device = _lib.cuda_device(problems[0][0].device, "BatchedStep")
         self.lr, self.beta1, self.beta2, self.eps = float(lr), float(betas[0]), float(betas[1]), float(eps)
         self.mode = loss_mode(loss)
         s = _lib.stream_handle(self.device)

@@ -11,11 +11,11 @@ libsvbrdf_hip.so:
 
 * :func:`imresize` -- ``cv2.resize(..., INTER_LANCZOS4)`` of an 8-bit image
   (src/imageio.py:75-76), ``svbrdf_resize_lanczos4_u8``;
-* :func:`imread` -- src/imageio.py:11-49: resize, ``u / 255`` and the
+* :func:`imread` -- src/imageio.py:11-51: resize, ``u / 255`` and the
   "srgb" / "rough" / "normal" flags, ``svbrdf_images_from_u8``; returns a
   float32 CUDA tensor [H, W, C] (or [H, W] for "rough"), as the reference
   returns a numpy array of that shape;
-* :func:`imwrite` -- src/imageio.py:52-72: clip per flag, ``* 255``,
+* :func:`imwrite` -- src/imageio.py:54-72: clip per flag, ``* 255``,
   truncate, ``svbrdf_images_to_u8``;
 * :func:`imconcat`, :func:`img9to1`, :func:`tex4to1` -- the montages of
   src/imageio.py:79-113.
@@ -243,7 +243,7 @@ def encode(images, flag=None):
 
 
 def imread(filename, flag=None, dim=None, device="cuda"):
-    """src/imageio.py:11-49.  Returns a float32 CUDA tensor [H, W, C] ([H, W]
+    """src/imageio.py:11-51.  Returns a float32 CUDA tensor [H, W, C] ([H, W]
     for "rough" and for grey images without a flag)."""
     u = read_png(filename)
     f = _flag(flag)
@@ -261,7 +261,7 @@ def imread(filename, flag=None, dim=None, device="cuda"):
 
 
 def imwrite(im, filename, flag=None, dim=None):
-    """src/imageio.py:52-72 for a float [H, W, C] or [H, W] image (CUDA tensor
+    """src/imageio.py:54-72 for a float [H, W, C] or [H, W] image (CUDA tensor
     or array).  The reference's optional float resize before writing is not
     used on its hot path and is not provided."""
     if dim is not None:

@@ -108,8 +108,8 @@ class FusedStep:
         self.renderer = renderer
         self.res, self.n_views = renderer.res, renderer.n_of_imgs
         dev = renderer.device
-        if textures.device.type != "cuda" or textures.dtype != torch.float32 or not textures.is_contiguous():
-            raise ValueError("textures must be a contiguous float32 CUDA tensor")
+        if textures.device != dev or textures.dtype != torch.float32 or not textures.is_contiguous():
+            raise ValueError(f"textures must be a contiguous float32 tensor on the renderer's device {dev}")
         if textures.numel() != 9 * self.res * self.res:
             raise ValueError(f"textures must hold 9x{self.res}x{self.res} values, got {tuple(textures.shape)}")
         if tuple(targets.shape) != (self.n_views, 3, self.res, self.res):
@@ -123,7 +123,7 @@ class FusedStep:
         self.optim_light = bool(optim_light)
         if self.optim_light:
             lp = renderer.light_pow
-            if lp.device != dev or lp.dtype != torch.float32 or not lp.is_contiguous() or lp.numel() != 3:
+            if _lib.cuda_device(lp.device) != dev or lp.dtype != torch.float32 or not lp.is_contiguous() or lp.numel() != 3:
                 raise ValueError("optim_light needs renderer.light_pow as a contiguous float32 [3] CUDA tensor")
             self.light_pow = lp
         else:
@@ -391,7 +391,7 @@ class SvbrdfOptim:
 
     @staticmethod
     def save_loss(losses, labels, save_dir, N):
-        """src/optimization.py:35-43: log(1 + loss) curves over N epochs.  Drawn
+        """src/optimization.py:35-42: log(1 + loss) curves over N epochs.  Drawn
         on a standalone Agg figure, so pyplot's global state and the caller's
         backend are left alone."""
         from matplotlib.backends.backend_agg import FigureCanvasAgg

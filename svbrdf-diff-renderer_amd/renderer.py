@@ -97,9 +97,7 @@ class Microfacet:
         self.f0 = 0.04
         self.eps = 1e-6
         self.size = float(size)
-        self.device = torch.device(device)
-        if self.device.type != "cuda":
-            raise RuntimeError(f"Microfacet on {self.device}: the HIP renderer runs on a GPU only")
+        self.device = _lib.cuda_device(device, "Microfacet")
         _lib.load()
         self.camera_pos = _as_device_f32(cl[0], self.device, "camera_pos")
         self.light_pos = _as_device_f32(cl[1], self.device, "light_pos")

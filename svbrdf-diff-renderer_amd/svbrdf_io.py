@@ -1,5 +1,5 @@
 """SvbrdfIO: the reference's map / target / calibration I/O object
-(src/svbrdf.py:85-237) with the per-pixel codec on the GPU.
+(src/svbrdf.py:86-221) with the per-pixel codec on the GPU.
 
 Same constructor (a JSON path and a device), attributes (reference_dir,
 target_dir, optimize_dir, rerender_dir, im_size, idx, n_of_imgs, camera_pos,
@@ -32,10 +32,20 @@ MAP_FILES = {"normal": "nom.png", "diffuse": "dif.png", "specular": "spe.png", "
 
 
 def _check_device(device):
-    dev = torch.device(device)
-    if dev.type != "cuda":
-        raise RuntimeError("svbrdf_amd runs on the GPU only (libsvbrdf_hip.so); got device %r" % (str(dev),))
-    return dev
+    return _lib.cuda_device(device, "SvbrdfIO")
+
+
+def calibration_arrays(camera_pos, light_pos, light_pow, idx):
+    """The host half of load_calibration_th (src/svbrdf.py:134-148): the
+    views listed in ``idx``, in that order, of camera_pos / light_pos ([N, 3],
+    cm) and the per-RGB light_pow [3], as contiguous float32 arrays."""
+    cam = np.asarray(camera_pos, np.float32)
+    light = np.asarray(light_pos, np.float32)
+    pow_ = np.asarray(light_pow, np.float32)
+    if cam.ndim != 2 or cam.shape[1] != 3 or light.shape != cam.shape or pow_.shape != (3,):
+        raise ValueError("camera_pos / light_pos must be [N, 3] and light_pow [3]")
+    sel = [int(i) for i in idx]
+    return np.ascontiguousarray(cam[sel, :]), np.ascontiguousarray(light[sel, :]), pow_
 
 
 def _sized(u8, res):
@@ -86,7 +96,7 @@ def reload_textures(textures, res):
 
 
 class SvbrdfIO:
-    """src/svbrdf.py:85-237."""
+    """src/svbrdf.py:86-221."""
 
     def __init__(self, json_dir, device):
         self.device = _check_device(device)
@@ -128,14 +138,11 @@ class SvbrdfIO:
 
     def load_calibration_th(self):
         """src/svbrdf.py:134-148."""
-        cam = np.array(self.camera_pos, "float32")[self.idx, :]
-        light = np.array(self.light_pos, "float32")[self.idx, :]
-        pow_ = np.array(self.light_pow, "float32")
-        self.cl = [self.np_to_th(np.ascontiguousarray(cam)), self.np_to_th(np.ascontiguousarray(light)),
-                   self.np_to_th(pow_)]
+        cam, light, pow_ = calibration_arrays(self.camera_pos, self.light_pos, self.light_pow, self.idx)
+        self.cl = [self.np_to_th(cam), self.np_to_th(light), self.np_to_th(pow_)]
 
     def load_textures_th(self, textures_dir, res):
-        """src/svbrdf.py:168-183 -> [1, 9, res, res] in [-1, 1]."""
+        """src/svbrdf.py:150-166 -> [1, 9, res, res] in [-1, 1]."""
         d = pathlib.Path(textures_dir)
         if not d.exists():
             raise FileNotFoundError(f"{d} does not exist")
@@ -145,7 +152,7 @@ class SvbrdfIO:
         return textures_from_maps_u8(maps["diffuse"], maps["normal"], maps["specular"], maps["roughness"], res)
 
     def save_textures_th(self, textures_th, textures_dir):
-        """src/svbrdf.py:185-207: nom / dif / spe / rgh.png and tex.png."""
+        """src/svbrdf.py:168-189: nom / dif / spe / rgh.png and tex.png."""
         d = pathlib.Path(textures_dir)
         d.mkdir(parents=True, exist_ok=True)
         dif, nom, spe, rgh = (m.cpu().numpy() for m in maps_u8_from_textures(textures_th))
@@ -155,7 +162,7 @@ class SvbrdfIO:
                     (d / "tex.png", montage_tex(nom, dif, spe, rgh))])
 
     def load_images_th(self, images_dir, res=256):
-        """src/svbrdf.py:209-221 -> [n_of_imgs, 3, res, res] in [0, 1]."""
+        """src/svbrdf.py:191-204 -> [n_of_imgs, 3, res, res] in [0, 1]."""
         d = pathlib.Path(images_dir)
         if not d.exists():
             raise FileNotFoundError(f"{d} does not exist")
@@ -167,7 +174,7 @@ class SvbrdfIO:
         return decode(torch.stack(ims), "srgb")
 
     def save_images_th(self, images_th, images_dir):
-        """src/svbrdf.py:223-237: <idx>.png per view (+ all.png for 9 views)."""
+        """src/svbrdf.py:206-221: <idx>.png per view (+ all.png for 9 views)."""
         if images_th.shape[0] != self.n_of_imgs:
             raise ValueError(f"expected {self.n_of_imgs} images, got {images_th.shape[0]}")
         d = pathlib.Path(images_dir)

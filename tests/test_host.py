@@ -13,12 +13,17 @@ from tests.conftest import ROOT
 
 
 def test_load_calibration_selects_views_in_idx_order(pkg):
-    from svbrdf_amd.calibration import load_calibration
-    cl, size, idx = load_calibration(ROOT / "tests" / "fixtures" / "calib_grid.json")
-    assert size == 12.5 and idx == [3, 0, 2]
-    np.testing.assert_array_equal(cl[0].numpy(), [[2, -2, 20], [-2, 2, 20], [-2, -2, 20]])
-    np.testing.assert_array_equal(cl[1].numpy()[:, 2], [21, 21, 21])
-    assert cl[2].dtype == torch.float32 and cl[2].tolist() == [900, 1000, 1100]
+    import json
+    from svbrdf_amd.svbrdf_io import calibration_arrays
+    data = json.loads((ROOT / "tests" / "fixtures" / "calib_grid.json").read_text())
+    assert data["im_size"] == 12.5 and data["idx"] == [3, 0, 2]
+    cam, light, pow_ = calibration_arrays(data["camera_pos"], data["light_pos"], data["light_pow"], data["idx"])
+    np.testing.assert_array_equal(cam, [[2, -2, 20], [-2, 2, 20], [-2, -2, 20]])
+    np.testing.assert_array_equal(light[:, 2], [21, 21, 21])
+    assert cam.dtype == light.dtype == pow_.dtype == np.float32 and pow_.tolist() == [900, 1000, 1100]
+    assert cam.flags["C_CONTIGUOUS"] and light.flags["C_CONTIGUOUS"]
+    with pytest.raises(ValueError):
+        calibration_arrays(data["camera_pos"], data["light_pos"], [1.0, 2.0], data["idx"])
 
 
 def test_synthetic_maps_follow_the_recipe(pkg):
@@ -141,7 +146,7 @@ def test_epoch_segments_cover_the_loop_and_end_at_save_points(pkg, epochs, save_
 
 
 def test_save_loss_writes_the_reference_plot(pkg, tmp_path):
-    """src/optimization.py:35-43: an 8 x 4 inch log(1 + loss) plot; drawn
+    """src/optimization.py:35-42: an 8 x 4 inch log(1 + loss) plot; drawn
     without touching pyplot's global state."""
     from PIL import Image
     import matplotlib
@@ -151,3 +156,65 @@ def test_save_loss_writes_the_reference_plot(pkg, tmp_path):
     with Image.open(tmp_path / "loss.jpg") as im:
         assert im.size == (800, 400)
     assert matplotlib.get_backend() == backend
+
+
+def _optim_on_host(pkg, res):
+    """SvbrdfOptim's initialisers are torch ops on self.device; on the host
+    they run without the library (the optimiser itself needs the GPU)."""
+    from types import SimpleNamespace
+    return pkg.SvbrdfOptim("cpu", SimpleNamespace(res=res))
+
+
+def test_init_from_const_encodings(pkg):
+    """src/svbrdf.py:23-30: u = dif / spe / rgh stored as t = 2u - 1 in the
+    channel order of src/svbrdf.py:166 (diffuse 0:3, normal xy 3:5, roughness
+    5, specular 6:9), normal xy = 0 (a flat normal); a leaf with grad."""
+    o = _optim_on_host(pkg, 8)
+    o.init_from_const()
+    t = o.textures
+    assert t.shape == (1, 9, 8, 8) and t.dtype == torch.float32 and t.requires_grad and t.is_leaf
+    x = t.detach()[0].numpy()
+    for c in range(9):
+        assert np.unique(x[c]).size == 1, c
+    v = x[:, 0, 0].astype(np.float64)
+    u = (v + 1) / 2
+    np.testing.assert_allclose(u[0:3], 0.5, atol=1e-7)
+    np.testing.assert_allclose(u[6:9], 0.04, atol=1e-7)
+    np.testing.assert_allclose(u[5], 0.2, atol=1e-7)
+    assert v[3] == 0.0 and v[4] == 0.0
+    # 0.5 -> exactly 0; 0.04 and 0.2 through float32 ops: within one ulp of 2u - 1
+    assert v[0] == 0.0
+    assert abs(v[6] - (-0.92)) <= np.spacing(np.float32(0.92)) and abs(v[5] - (-0.6)) <= np.spacing(np.float32(0.6))
+    o.init_from_const(dif=0.9, spe=0.3, rgh=0.7)
+    u = (o.textures.detach()[0, :, 3, 5].double().numpy() + 1) / 2
+    np.testing.assert_allclose(u[[0, 1, 2, 5, 6, 7, 8]], [0.9, 0.9, 0.9, 0.7, 0.3, 0.3, 0.3], atol=1e-7)
+
+
+def test_init_from_randn_statistics(pkg):
+    """src/svbrdf.py:32-39: normal xy ~ clamp(N(0,1)/4, -1, 1); diffuse, specular
+    and roughness u ~ clamp(N(0,1)/s + mean, 0, 1) with s = 8, 32, 16, stored as
+    2u - 1.  Checked through the moments of the clamped normal (the clamp
+    truncates specular at u = 0 for N < -1.28, ~10 % of its texels, which
+    gives exact -1 values), and determinism under torch.manual_seed."""
+    from scipy.stats import norm
+    R = 256
+    o = _optim_on_host(pkg, R)
+    torch.manual_seed(1234)
+    o.init_from_randn()
+    x = o.textures.detach()[0].double().numpy()
+    assert o.textures.requires_grad and x.min() >= -1 and x.max() <= 1
+    n = x[3:5]
+    assert abs(n.mean()) < 0.005 and abs(n.std() - 0.25) < 0.005
+    for ch, mean, s in ((slice(0, 3), 0.5, 8), (slice(6, 9), 0.04, 32), (slice(5, 6), 0.2, 16)):
+        u = (x[ch] + 1) / 2
+        a = -mean * s                      # N below a clamps to u = 0
+        p0 = norm.cdf(a)
+        # mean of clamp(N/s + mean, 0, 1) (the upper clamp is > 6 sigma away)
+        m = (mean * (1 - p0) + norm.pdf(a) / s)
+        assert abs(u.mean() - m) < 3e-3 / s * 8, (ch, u.mean(), m)
+        assert abs((u == 0).mean() - p0) < 0.01, (ch, (u == 0).mean(), p0)
+    assert (x[6:9] == -1.0).mean() > 0.09             # exact -1 specular texels
+    torch.manual_seed(1234)
+    o2 = _optim_on_host(pkg, R)
+    o2.init_from_randn()
+    assert torch.equal(o.textures, o2.textures)
