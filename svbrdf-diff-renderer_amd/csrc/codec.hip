@@ -1,7 +1,7 @@
 // codec.hip -- gfx950 kernels for the map / target codec on either side of the
 // optimisation path, and their extern "C" entry points (include/svbrdf_hip.h).
 //
-// Reference behaviour (src/imageio.py:11-76, src/svbrdf.py:126-132,168-237):
+// Reference behaviour (src/imageio.py:11-76, src/svbrdf.py:126-132,150-221):
 // PNG files hold 8-bit maps and images; loading resizes the 8-bit image with
 // cv2.INTER_LANCZOS4, divides by 255 and applies the per-map flag; saving
 // clips, scales by 255 and truncates to 8 bits.  The PNG container itself
@@ -14,7 +14,8 @@
 //   textures_from_u8_kernel 4 maps u8 -> [9, R, R] f32             10-12 B read, 36 B written / px
 //   textures_to_u8_kernel   [9, R, R] f32 -> 4 maps u8             36 B read, 10 B written / px
 //   images_to_u8_kernel     [n, C, h, w] f32 -> [n, h, w, C] u8    4 C B read, C B written / px
-//   upsample2x_kernel       [P, R, R] f32 -> [P, 2R, 2R] f32 bilinear (stage chaining in memory)
+//   upsample2x_kernel       [P, R, R] f32 -> [P, 2R, 2R] f32 bilinear (stage chaining in memory);
+//   upsample2x_x4_kernel    the same, 2 x 4 outputs per thread (even R, aligned destination rows)
 //
 // Float arithmetic follows the reference's float32 numpy / torch op order one
 // rounding per op: contraction into fused multiply-adds is off in this file,
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void images_from_u8_kernel(const uint8_t *_
     }
 }
 
-// load_textures_th (src/svbrdf.py:168-183) with imread flags (src/imageio.py:
+// load_textures_th (src/svbrdf.py:150-166) with imread flags (src/imageio.py:
 // 27-47): maps already at res x res, RGB bytes (roughness 1 or 3 channels).
 __global__ __launch_bounds__(kBlock) void textures_from_u8_kernel(const uint8_t *__restrict__ dif,
                                                                   const uint8_t *__restrict__ nom,
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void textures_from_u8_kernel(const uint8_t 
     for (int c = 0; c < 9; ++c) tex[(size_t)c * npix + p] = out[c];
 }
 
-// save_textures_th (src/svbrdf.py:185-207) with imwrite flags (src/imageio.py:
+// save_textures_th (src/svbrdf.py:168-189) with imwrite flags (src/imageio.py:
 // 52-72) for one texel: [9] floats -> dif / nom / spe [3] and rgh bytes.
 // The normal is SvbrdfIO.reconstruct_normal (src/svbrdf.py:126-132): z from
 // clamp(x^2 + y^2, 0, 1), divided by the 2-norm summed as (x^2 + y^2) + z^2.
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void textures_to_u8_kernel(const float *__r
     rgh[p] = r;
 }
 
-// imwrite before the PNG encode (src/imageio.py:52-72), for n images of
+// imwrite before the PNG encode (src/imageio.py:54-72), for n images of
 // h x w pixels: [n, C, h, w] float32 -> [n, h, w, C] bytes.  "srgb" and
 // "rough" clip to [0, 1]; "normal" clips to [-1, 1] and maps (n + 1) / 2;
 // without a flag the reference casts unclipped values, which is only defined
@@ -511,6 +512,55 @@ __global__ __launch_bounds__(kBlock) void upsample2x_kernel(const float *__restr
     }
 }
 
+// The same upsample, eight outputs per thread for an even res with 16-byte
+// aligned destination rows: source row i and source columns 2q, 2q + 1 give
+// output rows 2i, 2i + 1 and columns 4q .. 4q + 3.  With align_corners=False
+// and scale 2 the weights are 1/4 and 3/4 everywhere except output row /
+// column 0 (weights 1 and 0) and the neighbour index is clamped at the last
+// row / column, so a thread needs source rows i - 1, i, i + 1 and columns
+// 2q - 1 .. 2q + 2 (clamped): 12 loads and two float4 stores per plane
+// instead of 32 loads and eight scalar stores.  Every output is the same
+// expression, in the same order, as upsample2x_kernel evaluates it
+// (h0 (w0 a + w1 b) + h1 (w0 c + w1 d), one rounding per operation), so the
+// two kernels agree bit for bit.  blockIdx.y is the plane.
+__global__ __launch_bounds__(kBlock) void upsample2x_x4_kernel(const float *__restrict__ src, unsigned res,
+                                                               size_t src_stride, float *__restrict__ dst,
+                                                               size_t dst_stride) {
+    const unsigned half = res / 2u;
+    const unsigned t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= res * half) return;
+    const unsigned i = t / half, q = t - i * half;
+    const float *s = src + (size_t)blockIdx.y * src_stride;
+    float *d = dst + (size_t)blockIdx.y * dst_stride;
+    const unsigned last = res - 1u;
+    const unsigned c0 = q == 0u ? 0u : 2u * q - 1u, c1 = 2u * q, c2 = 2u * q + 1u,
+                   c3 = 2u * q + 2u > last ? last : 2u * q + 2u;
+    const bool left = q == 0u, top = i == 0u;
+    // horizontal pass of one source row: output columns 4q .. 4q + 3
+    auto hrow = [&](unsigned r, float (&h)[4]) {
+        const float *row = s + (size_t)r * res;
+        const float w0 = row[c0], w1 = row[c1], w2 = row[c2], w3 = row[c3];
+        h[0] = left ? 1.0f * w1 + 0.0f * w2 : 0.25f * w0 + 0.75f * w1;
+        h[1] = 0.75f * w1 + 0.25f * w2;
+        h[2] = 0.25f * w1 + 0.75f * w2;
+        h[3] = 0.75f * w2 + 0.25f * w3;
+    };
+    const unsigned rm = top ? 0u : i - 1u, rp = i + 1u > last ? last : i + 1u;
+    float hm[4], hi[4], hp[4];
+    hrow(rm, hm);
+    hrow(i, hi);
+    hrow(rp, hp);
+    float a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = top ? 1.0f * hi[k] + 0.0f * hp[k] : 0.25f * hm[k] + 0.75f * hi[k];
+        b[k] = 0.75f * hi[k] + 0.25f * hp[k];
+    }
+    const size_t o = (size_t)(2u * i) * (2u * res) + 4u * q;
+    *reinterpret_cast<float4 *>(d + o) = make_float4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<float4 *>(d + o + 2u * res) = make_float4(b[0], b[1], b[2], b[3]);
+}
+
 bool aligned(const void *p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
 
 unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -572,8 +622,19 @@ int svbrdf_upsample2x_bilinear(const float *src, int res, size_t src_stride, flo
     if (src_stride < n_in || dst_stride < n_out)
         return fail(SVBRDF_ESHAPE, "%s: plane strides %zu / %zu below %zu / %zu", what, src_stride, dst_stride, n_in,
                     n_out);
-    hipLaunchKernelGGL(upsample2x_kernel, dim3(blocks_for(n_out)), dim3(kBlock), 0, (hipStream_t)stream, src,
-                       (unsigned)res, src_stride, dst, dst_stride, n_planes);
+    // the kernels read src while they write dst: the two ranges must be disjoint
+    const uintptr_t s0 = (uintptr_t)src, s1 = s0 + ((n_planes - 1) * src_stride + n_in) * sizeof(float);
+    const uintptr_t d0 = (uintptr_t)dst, d1 = d0 + ((n_planes - 1) * dst_stride + n_out) * sizeof(float);
+    if (s0 < d1 && d0 < s1) return fail(SVBRDF_EINVAL, "%s: source and destination overlap", what);
+    hipStream_t s = (hipStream_t)stream;
+    if (res % 2 == 0 && dst_stride % 4 == 0 && aligned(dst, 16)) {
+        const unsigned threads = (unsigned)res * (unsigned)(res / 2);
+        hipLaunchKernelGGL(upsample2x_x4_kernel, dim3(blocks_for(threads), (unsigned)n_planes), dim3(kBlock), 0, s,
+                           src, (unsigned)res, src_stride, dst, dst_stride);
+    } else {
+        hipLaunchKernelGGL(upsample2x_kernel, dim3(blocks_for(n_out)), dim3(kBlock), 0, s, src, (unsigned)res,
+                           src_stride, dst, dst_stride, n_planes);
+    }
     return check_launch(what);
 }
 

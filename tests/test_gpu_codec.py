@@ -222,3 +222,50 @@ def test_upsample2x_padded_planes_and_validation(pkg):
         L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, 0, L.ptr(dst), 0, 65, stream)
     with pytest.raises(ValueError):
         pkg.upsample_textures(tex.cpu())
+
+
+@pytest.mark.parametrize("res", [2, 4, 6, 96, 512])
+def test_upsample2x_vector_path_equals_scalar_path(pkg, res):
+    """Even resolutions with aligned destination rows take the 2 x 4 outputs
+    per thread kernel; a destination one float off alignment takes the one
+    output per thread kernel.  Both evaluate the same expressions, so they
+    agree bit for bit, NaN and infinity propagation included (a NaN in the
+    first column or row passes through the weight-0 neighbour term, as in
+    torch)."""
+    L = pkg._lib
+    rng = np.random.default_rng(100 + res)
+    src = torch.tensor(rng.uniform(-1, 1, (9, res, res)).astype(np.float32), device=DEV)
+    src[0, 0, 1] = float("nan")
+    src[1, 1, 0] = float("inf")
+    src[2, res - 1, res - 1] = float("nan")
+    n_out = 4 * res * res
+    stream = L.stream_handle(DEV)
+    vec = torch.full((9 * n_out,), 7.0, device=DEV)
+    L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, 0, L.ptr(vec), 0, 9, stream)
+    buf = torch.full((9 * n_out + 4,), 7.0, device=DEV)
+    off = buf[1:1 + 9 * n_out]          # 4-byte offset: not 16-byte aligned
+    L.call("svbrdf_upsample2x_bilinear", L.ptr(src), res, 0, L.ptr(off), 0, 9, stream)
+    ref = torch.nn.functional.interpolate(src[None], scale_factor=2, mode="bilinear", align_corners=False)[0]
+    torch.cuda.synchronize()
+    a, b = vec.cpu().numpy(), off.cpu().numpy()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert float(buf[0]) == 7.0 and float(buf[-3]) == 7.0
+    r = ref.reshape(-1).cpu().numpy()
+    fin = np.isfinite(r)
+    assert np.array_equal(np.isnan(a), np.isnan(r)) and np.array_equal(np.isinf(a), np.isinf(r))
+    assert np.abs(a[fin] - r[fin]).max() <= 4e-7
+
+
+def test_upsample2x_refuses_overlapping_buffers(pkg):
+    L = pkg._lib
+    res = 8
+    buf = torch.zeros(9 * 5 * res * res, device=DEV)
+    stream = L.stream_handle(DEV)
+    with pytest.raises(ValueError, match="overlap"):
+        L.call("svbrdf_upsample2x_bilinear", L.ptr(buf), res, 0, L.ptr(buf), 0, 9, stream)
+    with pytest.raises(ValueError, match="overlap"):
+        # the destination starts inside the last source plane
+        L.call("svbrdf_upsample2x_bilinear", L.ptr(buf), res, 0, L.ptr(buf[8 * res * res + 4:]), 0, 9, stream)
+    # adjacent, not overlapping: accepted
+    L.call("svbrdf_upsample2x_bilinear", L.ptr(buf), res, 0, L.ptr(buf[9 * res * res:]), 0, 9, stream)
+    torch.cuda.synchronize()
