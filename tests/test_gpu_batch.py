@@ -6,12 +6,21 @@ apart after 3 steps), so its parameters must match running it alone through
 FusedStep bit for bit on >= 99 % of the values, to 1e-5 relative on the
 parameters and 1e-4 relative on the second moments (whose one-ulp gradient
 differences are squared), whatever the launch shape; losses are reduced per
-tile instead of per block, so they agree to rounding."""
+tile instead of per block, so they agree to rounding.
+
+Against the reference: the config-3 batch (64 materials, 24.6 Mpx) and a
+batch of uneven shapes, in several launch forms, each material's first step
+checked over its full maps as one iteration of the reference loop on that
+material alone (gradient against FP32 / FP64 torch autograd of the eager
+restatement, loss, every value of the Adam update)."""
 import numpy as np
 import pytest
 import torch
 
 from oracle import microfacet_np as M
+from oracle import microfacet_torch as MT
+from tests import parity
+from tests.test_gpu_step import check_adam_step, eager_loss_grad
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -149,3 +158,98 @@ def test_autotune_restores_state_and_picks_a_candidate(pkg, tmp_path, monkeypatc
     b.step()
     torch.cuda.synchronize()
     assert b.read_state()["step"] == 2.0
+
+
+# ---------------------------------------------------------------------------
+# Against the reference: each material's first step in the batch is one
+# iteration of the reference loop (src/svbrdf.py:56-71) on that material.
+
+PERSISTENT = {"tile_pixels": 4096, "pixels_per_thread": 2, "views_per_trip": 3, "blocks_per_cu": 0}
+GRID = {"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": -1}
+
+
+def first_step_in_batch(pkg, problems, cfg, loss):
+    """One batched step; per material (maps, exp_avg, exp_avg_sq) after it
+    (host copies) and the losses the batch recorded for it."""
+    bs = pkg.BatchedStep([(r, t.clone(), y) for r, t, y in problems], 0.01, loss=loss, config=cfg, history=1)
+    assert (bs.grid == 0) == (cfg["blocks_per_cu"] < 0)
+    bs.step()
+    torch.cuda.synchronize()
+    out = [(t.reshape(9, r.res, r.res).cpu().numpy(), m.reshape(9, r.res, r.res).cpu().numpy(),
+            v.reshape(9, r.res, r.res).cpu().numpy()) for r, t, m, v in zip(bs.renderers, bs.textures, bs.exp_avg,
+                                                                            bs.exp_avg_sq)]
+    losses = bs.losses()
+    del bs
+    torch.cuda.empty_cache()
+    return out, losses
+
+
+def check_batch_against_eager(pkg, specs, problems, results, loss, mode, label):
+    """Per material: the kernel's gradient (from Adam's first moment) against
+    FP32 / FP64 torch autograd of the eager restatement (on the GPU), under
+    tests/parity.py's criteria; its loss within 1e-5 of FP64; every value of
+    its Adam step against float64 Adam with that gradient."""
+    worst = 0.0
+    for i, spec in enumerate(specs):
+        res, V, size, cam, light, P, init, target = spec
+        L32, g32 = eager_loss_grad(init, target, res, size, cam, light, P, mode, torch.float32, DEV)
+        L64, g64 = eager_loss_grad(init, target, res, size, cam, light, P, mode, torch.float64, DEV)
+        for form, (per_mat, losses) in results.items():
+            p1, m1, v1 = per_mat[i]
+            g = m1.astype(np.float64) / np.float64(np.float32(0.1))
+            tag = f"{label} {form} material {i} ({res}^2 x {V})"
+            worst = max(worst, max(parity.check_gradient(g, g32, g64, tag)))
+            assert abs(losses[i] - L64) <= 1e-5 * abs(L64), (tag, losses[i], L64)
+            z = np.zeros_like(init)
+            check_adam_step(init, z, z, g, 0, p1, m1, v1, label=tag)
+        torch.cuda.empty_cache()
+    return worst
+
+
+@pytest.mark.timeout(900)
+def test_config3_batch_against_the_reference(pkg):
+    """Config 3 as bench.py runs it: 64 materials (24 at 256^2, 24 at 512^2,
+    16 at 1024^2), views drawn from 4..16, flash positions jittered +-3 cm
+    around the 3x3 grid (bench.batch_materials, seed 3), log-L2.  One batched
+    step in each launch form -- one block per 256-pixel tile (the autotuned
+    form on this batch) and a persistent grid drawing 4,096-pixel tiles, two
+    pixels per thread, three views per loop trip -- and every material's
+    update checked over its full maps against one iteration of the
+    reference loop on that material alone."""
+    import bench
+    syn = pkg.synthetic
+    mats = bench.batch_materials(pkg, DEV)
+    assert sorted(m[0] for m in mats) == [256] * 24 + [512] * 24 + [1024] * 16
+    assert min(m[1] for m in mats) >= 4 and max(m[1] for m in mats) <= 16
+    P, size = syn.light_power(), syn.PLANE_SIZE_CM
+    specs, problems = [], []
+    for res, V, gt, init, cam, light in mats:
+        cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+        with torch.no_grad():
+            tgt = MT.MicrofacetEager(res, V, size, cl, DEV, torch.float32).eval(torch.tensor(gt, device=DEV)[None])
+        r = pkg.Microfacet(res, V, size, cl, DEV)
+        problems.append((r, torch.tensor(init, device=DEV)[None].contiguous(), tgt))
+        specs.append((res, V, size, cam, light, P, init, tgt))
+    results = {name: first_step_in_batch(pkg, problems, cfg, "log_l2")
+               for name, cfg in (("grid", GRID), ("persistent", PERSISTENT))}
+    worst = check_batch_against_eager(pkg, specs, problems, results, "log_l2", MT.LOSS_LOG_L2, "config 3")
+    print(f"config 3: worst per-channel |g - g64|/|g64| over 64 materials x 2 forms {worst:.2e}")
+
+
+@pytest.mark.parametrize("loss,mode", [("log_l2", MT.LOSS_LOG_L2), ("l2", MT.LOSS_L2)])
+def test_uneven_batch_against_the_reference(pkg, loss, mode):
+    """Odd sizes (33^2, 100^2: tiles that end inside a tile), one view,
+    sixteen views and a material whose lights sit off its cameras (the
+    general path), in three launch forms, against the reference loop on
+    each material alone."""
+    specs = make_problems(pkg, SHAPES, seed=30)
+    problems = []
+    for res, V, size, cam, light, P, init, target in specs:
+        cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+        problems.append((pkg.Microfacet(res, V, size, cl, DEV), torch.tensor(init, device=DEV)[None].contiguous(),
+                         torch.tensor(target, device=DEV)))
+    forms = (("grid", GRID), ("persistent", PERSISTENT),
+             ("persistent-1px", {"tile_pixels": 1024, "pixels_per_thread": 1, "views_per_trip": 1,
+                                 "blocks_per_cu": 1}))
+    results = {name: first_step_in_batch(pkg, problems, cfg, loss) for name, cfg in forms}
+    check_batch_against_eager(pkg, specs, problems, results, loss, mode, "uneven batch")

@@ -39,13 +39,17 @@ def feature_grad(pkg, renderer_cls, tex0, gt, cam, light, P, size, dtype):
     return L.item(), t.grad[0].double().cpu().numpy()
 
 
-def test_feature_loss_gradient_through_hip_adjoint(pkg):
-    gt, init, cam, light, P, size = setup(pkg)
+@pytest.mark.parametrize("res", [32, 256])
+def test_feature_loss_gradient_through_hip_adjoint(pkg, res):
+    """Config 4's workload shape (256^2 x 9 views, VGG-19-shaped stack to
+    conv5_1, Gram L2 on 5 layers + pixel L2) and a small case."""
+    gt, init, cam, light, P, size = setup(pkg, res=res)
     L, g = feature_grad(pkg, None, init, gt, cam, light, P, size, torch.float32)
     L32, g32 = feature_grad(pkg, MT.MicrofacetEager, init, gt, cam, light, P, size, torch.float32)
     L64, g64 = feature_grad(pkg, MT.MicrofacetEager, init, gt, cam, light, P, size, torch.float64)
     assert abs(L - L64) <= 1e-4 * abs(L64)
-    parity.check_gradient(g, g32, g64, "feature loss")
+    errs = parity.check_gradient(g, g32, g64, f"feature loss {res}^2")
+    print(f"feature loss {res}^2 x 9: per-channel |g - g64|/|g64| max {max(errs):.2e}")
 
 
 def test_feature_loss_optimisation_runs(pkg):

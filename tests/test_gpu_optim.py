@@ -190,3 +190,22 @@ def test_scripts_materialgan_drivers(pkg, tmp_path):
     assert torch.equal(latent.to(DEV), opt.latent.detach())
     assert len(torch.load(run / "3" / "optim_noise.pt", weights_only=True)) == len(opt.noises)
     assert (sio.optimize_dir / "tex.png").is_file() and (sio.rerender_dir / "00.png").is_file()
+
+
+def test_device_without_index(pkg, tmp_path):
+    """Microfacet(..., "cuda") -- the reference's callers pass a device such
+    as "cuda" or th.device("cuda:0") -- holds the resolved device, so the
+    optimiser, the light-power path and the batch accept its tensors."""
+    res, V = 16, 9
+    init, cl, targets = problem(pkg, res, V)
+    r = pkg.Microfacet(res, V, pkg.synthetic.PLANE_SIZE_CM, [c.clone() for c in cl], "cuda")
+    assert r.device == torch.device("cuda", torch.cuda.current_device())
+    opt = pkg.SvbrdfOptim("cuda", r)
+    opt.init_from_tex(torch.tensor(init, device="cuda")[None])
+    opt.load_targets(targets)
+    losses = opt.optim(5, 0.01, Recorder(tmp_path, [c.clone() for c in cl]), optim_light=True)
+    assert len(losses) == 5 and np.all(np.isfinite(losses))
+    b = pkg.BatchedStep([(r, torch.tensor(init, device="cuda")[None].contiguous(), targets)], 0.01)
+    b.step()
+    torch.cuda.synchronize()
+    assert np.isfinite(b.losses()).all()
