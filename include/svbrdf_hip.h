@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 3
+#define SVBRDF_ABI_VERSION 4
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -118,6 +118,22 @@ int svbrdf_render_backward(const float *textures, int res, int n_views, float si
                            const float *light_pow, const float *grad_images,
                            float *grad_textures, float *grad_light_pow,
                            void *workspace, size_t workspace_bytes, void *stream);
+
+/* svbrdf_render_backward without a workspace argument and without any
+ * allocation: no counterpart in the reference.  It is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The light-power partial
+ * sums go to a 1 MiB device array inside the library (one per GPU, held
+ * until the process ends), so res is limited to 4096 (larger res returns
+ * SVBRDF_ESHAPE: use svbrdf_render_backward with a workspace).  The first
+ * call on a device resolves that array's address (hipGetSymbolAddress, no
+ * allocation).  All calls on one GPU share the array: calls that overlap on
+ * one GPU (concurrent streams) race on it and must be ordered by the caller.
+ * Results are bit-identical to svbrdf_render_backward. */
+int svbrdf_render_backward_noalloc(const float *textures, int res, int n_views, float size,
+                                   const float *camera_pos, const float *light_pos,
+                                   const float *light_pow, const float *grad_images,
+                                   float *grad_textures, float *grad_light_pow, void *stream);
 
 /* Target preparation for the fused step (once per target set).  For
  * SVBRDF_LOSS_LOG_L2 writes ln(clamp(targets, 1e-6^(1/2.2), 1)); for

@@ -13,7 +13,7 @@ _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libsvbrdf_hip.so"
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK = 0
 EINVAL = -1
@@ -44,6 +44,8 @@ SIGNATURES = {
                                            ctypes.c_int, _vp, _vp]),
     "svbrdf_render_backward_terms": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp, _vp,
                                                     ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "svbrdf_render_backward_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _vp,
+                                                      _vp, _vp, _vp, _vp, _vp]),
     "svbrdf_prepare_targets": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "svbrdf_step_state_init": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp]),
     "svbrdf_fused_step_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1106,6 +1107,50 @@ int check_variant(const char *what, int loss_mode, int npix, int unroll) {
     return SVBRDF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free adjoint.  The light-power partial sums (one float4 slot per
+// 256-pixel block) of svbrdf_render_backward_noalloc live in this file-scope
+// device array instead of a caller workspace: 65,536 slots, 1 MiB, enough for
+// res <= 4096.  Each GPU holds its own copy for the life of the process.  In
+// host code `scratch` names a host-side stand-in, not device memory: the
+// device address comes from hipGetSymbolAddress, once per device (with that
+// device current), and only that address reaches kernels.
+constexpr int kScratchMaxRes = 4096;
+constexpr unsigned kScratchSlots = (unsigned)(((size_t)kScratchMaxRes * kScratchMaxRes + kBlock - 1) / kBlock);
+constexpr size_t kScratchBytes = (size_t)kScratchSlots * 4 * sizeof(float);
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScratchSlots * 4];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_address[kMaxDevices];
+
+// Device address of the current device's `scratch`, resolved on the first
+// call on that device.
+int scratch_address(const char *what, void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return fail(SVBRDF_EHIP, "%s: no current HIP device (or device index >= %d)", what, kMaxDevices);
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(SVBRDF_EHIP, "%s: hipGetSymbolAddress(scratch) on device %d failed: %s", what, dev,
+                        hipGetErrorString(e));
+        }
+        g_scratch_address[dev] = p;
+    }
+    *out = g_scratch_address[dev];
+    return SVBRDF_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1203,6 +1248,20 @@ int svbrdf_render_backward(const float *textures, int res, int n_views, float si
     return svbrdf_render_backward_terms(textures, res, n_views, size, camera_pos, light_pos, light_pow,
                                         SVBRDF_TERMS_EVAL, grad_images, grad_textures, grad_light_pow, workspace,
                                         workspace_bytes, stream);
+}
+
+int svbrdf_render_backward_noalloc(const float *textures, int res, int n_views, float size, const float *camera_pos,
+                                   const float *light_pos, const float *light_pow, const float *grad_images,
+                                   float *grad_textures, float *grad_light_pow, void *stream) {
+    const char *what = "svbrdf_render_backward_noalloc";
+    if (res > kScratchMaxRes)
+        return fail(SVBRDF_ESHAPE, "%s: res=%d above %d, the largest its %zu-byte scratch array serves (use "
+                    "svbrdf_render_backward with a workspace)", what, res, kScratchMaxRes, kScratchBytes);
+    void *ws = nullptr;
+    if (int rc = scratch_address(what, &ws)) return rc;
+    return svbrdf_render_backward_terms(textures, res, n_views, size, camera_pos, light_pos, light_pow,
+                                        SVBRDF_TERMS_EVAL, grad_images, grad_textures, grad_light_pow, ws,
+                                        kScratchBytes, stream);
 }
 
 int svbrdf_prepare_targets(const float *targets, size_t count, int loss_mode, float *prepared, void *stream) {

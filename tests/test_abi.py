@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 3 == pkg._lib.ABI_VERSION
+    assert lib.svbrdf_abi_version() == 4 == pkg._lib.ABI_VERSION
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
@@ -155,3 +155,11 @@ def test_sizes_beyond_the_limits_are_refused(lib):
     assert lib.svbrdf_render(_fake(1), 8, lib.svbrdf_max_views() + 1, ctypes.c_float(17.0), _fake(2), _fake(3),
                              _fake(4), _fake(5), None) == -2
     assert _step(lib, res=big) == -2
+
+
+def test_noalloc_adjoint_refuses_sizes_beyond_its_scratch(lib):
+    """svbrdf_render_backward_noalloc serves res <= 4096 (its 1 MiB array);
+    larger sizes are refused on the host before any HIP call."""
+    f = _fake()
+    assert lib.svbrdf_render_backward_noalloc(f, 4097, 9, 17.0, f, f, f, f, f, f, None) == -2
+    assert b"4096" in lib.svbrdf_last_error()
