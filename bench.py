@@ -53,6 +53,7 @@ import json
 import math
 import os
 import pathlib
+import re
 import sys
 import time
 
@@ -68,11 +69,29 @@ METRIC = "per-pixel optim iters/sec at 1024²×9 views (fused vs torch autograd)
 HBM_PEAK_GBS = 8000.0
 
 
-# HBM bytes per fused-step launch read from the profiler (rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE, in KiB per dispatch, FETCH_SIZE scaled by 2.0 as a
-# plain device copy of known size calibrates it, WRITE_SIZE by 1.0), for the
-# shapes that were profiled: 110,690 KiB x 2 + 110,852 KiB at 1024^2 x 9.
-MEASURED_TRAFFIC = {(1024, 9): (110690 * 2 + 110852) * 1024}
+# HBM bytes per fused-step launch, from the rocprofv3 summary committed under
+# profiles/ for the profiled shape (FETCH_SIZE and WRITE_SIZE in KiB per
+# dispatch, each scaled by the factor a plain device copy of known size gives
+# the same counter; the summary states both factors).
+TRAFFIC_PROFILES = {(1024, 9): ROOT / "profiles" / "fused_step_traffic_1024x9.md"}
+
+
+def measured_traffic(res, views):
+    """(bytes per launch, source) from the committed traffic profile of this
+    shape, or (None, reason)."""
+    path = TRAFFIC_PROFILES.get((res, views))
+    if path is None or not path.exists():
+        return None, f"no traffic profile for {res}^2 x {views}"
+    text = path.read_text()
+    row = re.search(r"\|\s*`fused_step_kernel<1, false>`\s*\|\s*([\d,]+)\s*\|\s*([\d,]+)\s*\|", text)
+    scale = re.search(r"FETCH_SIZE is scaled \S(\d+(?:\.\d+)?) and WRITE_SIZE \S(\d+(?:\.\d+)?)", text)
+    if not row or not scale:
+        return None, f"{path.name}: no fused_step_kernel row or scale factors"
+    fetch, write = (int(x.replace(",", "")) for x in row.groups())
+    fs, ws = (float(x) for x in scale.groups())
+    nbytes = int(round((fetch * fs + write * ws) * 1024))
+    return nbytes, (f"profiles/{path.name}: FETCH_SIZE {fetch:,} KiB x {fs:g} + WRITE_SIZE {write:,} KiB x {ws:g} "
+                    "per dispatch (rocprofv3 --pmc)")
 
 
 def algorithmic_bytes_step(res, views):
@@ -232,7 +251,7 @@ class EagerReferencePath:
         return img.clamp(1e-6, 1) ** (1 / 2.2)
 
 
-def torch_gpu_baseline(prob, res, views, dev, steps=5, warmup=2, sync=True):
+def torch_gpu_baseline(prob, res, views, dev, steps=20, warmup=3, sync=True):
     path = EagerReferencePath(res, views, 17.0, prob["cam"], prob["light"], prob["P"], dev)
     tex = torch.tensor(prob["init"], device=dev)[None].requires_grad_(True)
     log_tgt = torch.log(prob["target"].detach().clamp(1e-6 ** (1 / 2.2), 1))
@@ -311,8 +330,14 @@ def batch_leg(pkg, dev, steps=20, warmup=3):
         g.replay()
     ms_loop = time_events(g.replay, steps)
     total_px = sum(res * res for res, *_ in mats)
+    if cfg["blocks_per_cu"] < 0:
+        form = "one block per 256-pixel tile in cost order (the dispatcher balances the load)"
+    else:
+        form = (f"persistent grid of {cfg['blocks_per_cu']} blocks per CU drawing {cfg['tile_pixels']}-pixel tiles "
+                "from a ticket counter")
     return {"workload": f"{len(mats)} materials (24x256^2, 24x512^2, 16x1024^2 = {total_px} px), V in 4..16, "
-                        "fused log-L2 iteration, one launch per iteration (autotuned form), HIP graphs of 10 iterations",
+                        f"fused log-L2 iteration; autotuned form: {form}; three launches per iteration (the step, "
+                        "the per-material loss reduction, the state advance), HIP graphs of 10 iterations",
             "value": round(1000.0 / ms, 2), "unit": "batched it/s", "ms_per_step": round(ms, 4),
             "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
             "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes,
@@ -346,10 +371,13 @@ def staged_leg(pkg, dev, views, stages=(256, 512, 1024), iters=20):
     tex = None
     out = {"stages": {}}
     total = 0.0
+    # one material: the finest ground truth, box-averaged for the coarse stages
+    finest = max(stages)
+    gt_fine = syn.ground_truth_textures(finest, seed=0)
     for res in stages:
         cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
         r = pkg.Microfacet(res, views, syn.PLANE_SIZE_CM, cl, dev)
-        gt = torch.tensor(syn.ground_truth_textures(res, seed=0), device=dev)[None]
+        gt = torch.tensor(syn.downsample(gt_fine, finest // res), device=dev)[None]
         with torch.no_grad():
             target = r.eval(gt)
         if tex is None:
@@ -370,11 +398,16 @@ def staged_leg(pkg, dev, views, stages=(256, 512, 1024), iters=20):
         losses = fs.losses()
         out["stages"][f"{res}^2"] = {"ms": round(ms, 4), "it_per_s": round(iters * 1000.0 / ms, 1),
                                      "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+        first = out["stages"][f"{stages[0]}^2"]["loss_first"]
+        if res != stages[0] and not losses[0] < first:
+            raise RuntimeError(f"staged leg: the {res}^2 stage starts at loss {losses[0]:.4g}, not below the "
+                               f"{stages[0]}^2 initial loss {first:.4g}: the stages are not one material")
         total += ms
     out["total_ms"] = round(total, 4)
     out["what"] = (f"{iters} fused log-L2 iterations per stage at {', '.join(f'{r}^2' for r in stages)} x {views} "
-                   "views, bilinear x2 between stages, one graph replay per stage (first replay: includes "
-                   "the graph launch)")
+                   f"views on one material ({finest}^2 ground truth, box-averaged for the coarse stages), "
+                   "bilinear x2 between stages, one graph replay per stage (first replay: includes the graph "
+                   "launch); each later stage starts below the first stage's initial loss")
     return out
 
 
@@ -608,13 +641,32 @@ def compose_leg(pkg, dev, res=256, views=9, iters=100):
             "loss_first": first, "loss_last": last}
 
 
+def host_cpu():
+    """(threads to use, description) for the CPU legs: every core of this
+    process's affinity set, capped at OMP_NUM_THREADS when the environment
+    sets it (the GPU box allots each GPU a share of its host cores that
+    way), and the CPU model from /proc/cpuinfo."""
+    affinity = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, {"cpu_model": model, "affinity_cores": affinity, "omp_num_threads": omp}
+
+
 def cpu_forward(res, views, n=3):
     """One forward render of the eager restatement on the host cores at the
     bench shape (the CPU counterpart of the forward line): median of n."""
     from oracle import microfacet_torch as MT   # the CPU baseline leg only
     pkg = __graft_entry__.load_package()
     syn = pkg.synthetic
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    threads, host = host_cpu()
     torch.set_num_threads(threads)
     cam, light = syn.flash_grid(n_views=views)
     cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(syn.light_power())]
@@ -627,7 +679,7 @@ def cpu_forward(res, views, n=3):
             r.eval(gt)
             times.append(time.perf_counter() - t0)
     med = float(np.median(times))
-    return {"value": round(1.0 / med, 3), "unit": "renders/s", "cores": threads, "kind": "port",
+    return {"value": round(1.0 / med, 3), "unit": "renders/s", "cores": threads, "kind": "port", **host,
             "sample": f"{n} forward renders of the eager restatement at {res}^2 x {views} views, median s = "
                       f"{med:.3f}"}
 
@@ -638,7 +690,7 @@ def cpu_baseline(res, views, seconds):
     from oracle import microfacet_torch as MT   # the CPU baseline leg only
     pkg = __graft_entry__.load_package()
     syn = pkg.synthetic
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    threads, host = host_cpu()
     torch.set_num_threads(threads)
     gt = syn.ground_truth_textures(res, seed=0)
     init = syn.init_textures(gt, seed=1)
@@ -666,7 +718,7 @@ def cpu_baseline(res, views, seconds):
             break
     timed = times[1:] if len(times) > 1 else times
     s = float(np.median(timed))
-    return {"value": 1.0 / s, "unit": "it/s", "cores": threads, "kind": "port",
+    return {"value": 1.0 / s, "unit": "it/s", "cores": threads, "kind": "port", **host,
             "sample": f"{len(timed)} timed iteration(s) (+1 warm-up) of the eager restatement at {res}^2 x "
                       f"{views} views, log-L2 + torch.optim.Adam, median s/it = {s:.3f}"}
 
@@ -752,6 +804,12 @@ def main():
 
     nbytes = algorithmic_bytes_step(res, views)
     achieved = nbytes / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_source = measured_traffic(res, views)
+    ws_bytes = working_set_bytes(fs, views)
+    regime = ("infinity-cache-resident: the working set between iterations fits the 256 MiB Infinity Cache, so "
+              "part of each iteration's reads come from it; hbm_resident_frac is the same step with its working "
+              "set kept out of the cache (cache_cold)" if ws_bytes <= INFINITY_CACHE_BYTES else
+              "hbm-resident: the working set exceeds the 256 MiB Infinity Cache")
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "it/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
@@ -763,7 +821,8 @@ def main():
                    "graph_steps": None if args.eager else args.graph_steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((res, views)),
+                     "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "algorithmic_bytes": nbytes, "kernel": "fused_step_kernel",
                      "kernel_ms": round(kernel_ms, 5),
                      "step_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -771,8 +830,9 @@ def main():
                      # targets at their plane strides): at or below the 256 MB
                      # Infinity Cache, part of each iteration's reads are served
                      # from it, and `traffic` (L2-to-fabric bytes) cannot tell them apart
-                     "working_set_bytes": working_set_bytes(fs, views),
-                     "infinity_cache_bytes": INFINITY_CACHE_BYTES},
+                     "working_set_bytes": ws_bytes,
+                     "infinity_cache_bytes": INFINITY_CACHE_BYTES,
+                     "regime": regime, "hbm_resident_frac": None},
         "loss_after": loss_after,
         "per_replay_us_per_step": spread,
     }
@@ -814,6 +874,7 @@ def main():
         out["optim_light"] = light_leg(pkg, dev, res, views)
     if not args.no_cache_cold:
         out["cache_cold"] = cache_cold_leg(pkg, dev, res, views)
+        out["roofline"]["hbm_resident_frac"] = out["cache_cold"]["frac"]
     out["hbm_copy"] = copy_bandwidth(dev)
     if not args.no_staged:
         out["staged"] = staged_leg(pkg, dev, views)
