@@ -71,9 +71,12 @@ struct Maps {
     float Fc[3];   // Fresnel term when light and camera coincide (v.h = 1)
 };
 
-// 2^((A v.h + B) v.h) at v.h = 1 (co-located light and camera).
-
-__device__ __forceinline__ float sphg_coloc() { return fexp2(kFresA + kFresB); }
+// 2^((A v.h + B) v.h) at v.h = 1 (co-located light and camera): 2 to the
+// fp32 sum -12.5378895 of the two constants, correctly rounded (the value
+// torch.pow gives for the reference's expression at v.h = 1), a constant
+// instead of one v_exp_f32 per pixel.
+constexpr float kSphgColoc = 0x1.60a77cp-13f;
+__device__ __forceinline__ float sphg_coloc() { return kSphgColoc; }
 
 // u^2.2 of u = (t+1)/2; u in {0} or >= 2^-25, so log2 is finite or -inf.
 __device__ __forceinline__ float map_pow(float t) {
@@ -297,7 +300,7 @@ __device__ __forceinline__ void shade_backward(const Maps &m, const Geom &g, con
 // pixel constant Maps::Fc, and the three dot-product gradients act along v.
 struct ShadeC {
     float v[3];
-    float nv_raw, nv, c2, den, iqd, D, idv, g1, G, iq, DGq, inv_d2, nl_d2;
+    float nv_raw, nv, c2, den, iqd, D, idv, g1, G, iq, Dq1, DGq, inv_d2, nl_d2;
 };
 
 __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, float cx, float cy, float cz,
@@ -306,7 +309,10 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     const float sv = vx * vx + vy * vy + vz * vz;
     const float rv = frsq(sv);
     s.v[0] = vx * rv; s.v[1] = vy * rv; s.v[2] = vz * rv;
-    s.inv_d2 = frcp(sv);
+    // 1/d^2 as the square of the 1/d that normalises v, as the forward
+    // render computes it: the adjoint's recomputed image, and so its clamp
+    // mask, follows the forward's rounding
+    s.inv_d2 = rv * rv;
     s.nv_raw = m.n[0] * s.v[0] + m.n[1] * s.v[1] + m.n[2] * s.v[2];
     s.nv = tmax0(s.nv_raw);
     // GGX with 1 - (n.h)^2 = |v - (n.v) n|^2 for unit n, v: the components
@@ -325,7 +331,9 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     s.g1 = s.nv * s.idv;
     s.G = s.g1 * s.g1;
     s.iq = frcp(4.0f * s.nv * s.nv + kEps);
-    s.DGq = s.D * s.G * s.iq;
+    // D G / q as (D g1 / q) g1: the first factor is the adjoint's D G /(q n.v)
+    s.Dq1 = s.D * s.iq * s.g1;
+    s.DGq = s.Dq1 * s.g1;
     s.nl_d2 = s.nv * s.inv_d2;
 }
 
@@ -360,7 +368,7 @@ __device__ __forceinline__ void shade_coloc_fwd(const Maps &m, float px, float p
 // Fresnel factors paired with gf: F_c, or P_c F_c when the caller passes
 // gf / P_c (and scales a.S and a.T by P_c after the view loop).
 __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, const float gf[3], float gnl,
-                                               float rnv, const float Fw[3], Accum &a) {
+                                               const float Fw[3], Accum &a) {
     float W;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -369,16 +377,19 @@ __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, c
         W = c == 0 ? gf[c] * Fw[c] : W + gf[c] * Fw[c];
     }
     // f2 = D G F / q with D, G, q functions of n.v (= n.l = n.h) and W =
-    // sum_c dL/df_c F_c; WD = W D G / q.  With rnv = 1/n.v (also used for
-    // the direct n.l term), den and iqd of GGX and idv of Smith:
+    // sum_c dL/df_c F_c; WD = W D G / q.  With den and iqd of GGX and idv of
+    // Smith:
     //   d ln D / d nv = -4 pi (a2 - 1) den iqd nv    (dden/dnv = 2 (a2 - 1) nv)
     //   d ln G / d nv = 2 k idv / nv                 (G = g1^2, g1 = nv idv)
     //   d ln q / d nv = 8 nv iq
     //   dL/dk        = -2 WD (1 - nv) idv            (d ln g1/dk = -(1 - nv) idv)
     //   dL/da2       = W G iq iqd (1 - 2 pi a2 den c2 iqd)
+    // The G term WD 2 k idv / nv is 2 k idv^2 W (D g1 / q): no 1/n.v.
     const float WD = W * s.DGq;
     const float di = s.den * s.iqd;
-    float gnv = gnl + WD * (s.nv * ((-4.0f * kPi) * (m.a2 - 1.0f) * di - 8.0f * s.iq) + 2.0f * m.k * s.idv * rnv);
+    const float WDq1 = W * s.Dq1;
+    float gnv = gnl + WD * (s.nv * ((-4.0f * kPi) * (m.a2 - 1.0f) * di - 8.0f * s.iq)) +
+                (2.0f * m.k) * (s.idv * s.idv) * WDq1;
     const float ga2 = W * s.iq * s.G * s.iqd * (1.0f - (2.0f * kPi) * m.a2 * di * s.c2);
     a.A += ga2 * 2.0f * m.alpha - WD * (1.0f - s.nv) * s.idv;
     gnv = s.nv_raw >= 0.0f ? gnv : 0.0f;

@@ -301,20 +301,24 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
             PF[c] = P[c] * m.Fc[c];
         }
     }
-    // dL/dimages of view k + 1 are loaded while view k is shaded
+    // dL/dimages of view k + 1 are loaded while view k is shaded.  The loads
+    // are unconditional (no exec-mask branches around them): a lane past the
+    // end reads the last pixel, which it owns as a copy and never stores, and
+    // the last view re-reads its own values.
     auto load_gout = [&](int k, float (&g)[3]) {
         const Tensor G = tensor(gout + (size_t)(3 * k) * npix, 3, npix);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) g[c] = px.valid ? ld(G, c, px.off) : 0.0f;
+        for (int c = 0; c < 3; ++c) g[c] = ld(G, c, px.off);
     };
     float gnext[3];
     load_gout(0, gnext);
+#pragma unroll 2
     for (int k = 0; k < n_views; ++k) {
         const ViewPos vp = load_view(cam, light, k);
         float go[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) go[c] = gnext[c];
-        if (k + 1 < n_views) load_gout(k + 1, gnext);
+        load_gout(k + 1 < n_views ? k + 1 : k, gnext);
         // d/dimg of clamp(img)^(1/2.2) = (1/2.2) out / img inside [eps, 1]
         auto dimg = [&](float img, float g) {
             const float ic = tclamp(img, kEps, 1.0f);
@@ -335,9 +339,10 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
                 const float gimg = ic == img ? go[c] * fexp2(flog2(ic) * (kInvGamma - 1.0f)) : 0.0f;
                 gf[c] = gimg * s.nl_d2;
                 gnl += gimg * pf;
-                if (LIGHT_GRAD) a.gP[c] += gimg * (m.f1[c] + s.DGq * m.Fc[c]) * s.nl_d2;
+                // dL/dP_c = sum gimg f_c nl_d2 = (sum gimg img_c) / P_c
+                if (LIGHT_GRAD) a.gP[c] += gimg * img;
             }
-            backward_coloc(m, s, gf, gnl * s.inv_d2, frcp(fmaxf(s.nv, 1e-30f)), PF, a);
+            backward_coloc(m, s, gf, gnl * s.inv_d2, PF, a);
         } else {
             Geom g;
             geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
@@ -356,7 +361,9 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
             a.S[c] *= kInvGamma * P[c];
             a.T[c] *= kInvGamma * P[c];
             a.gn[c] *= kInvGamma;
-            if (LIGHT_GRAD) a.gP[c] *= kInvGamma;
+            // P_c = 0 renders img = 0 < eps everywhere: the clamp masks every
+            // view, and the gradient is 0
+            if (LIGHT_GRAD) a.gP[c] = P[c] != 0.0f ? a.gP[c] * kInvGamma / P[c] : 0.0f;
         }
         a.A *= kInvGamma;
     }
@@ -390,7 +397,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_B
     if (LIGHT_GRAD) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float s = block_sum(a.gP[c], lds);
+            // a lane past the end shaded a copy of the last pixel: no contribution
+            const float s = block_sum(px.valid ? a.gP[c] : 0.0f, lds);
             if (threadIdx.x == 0) light_partials[(size_t)blockIdx.x * 4 + c] = s;
         }
     }
@@ -626,7 +634,7 @@ __device__ __forceinline__ void fused_views_coloc(const Pix &px, const Maps &m, 
             if (OPT_LIGHT) rP[c] += rm;
         }
         const float rnv = frcp(fmaxf(s.nv, 1e-30f));
-        backward_coloc(m, s, gf, rsum * rnv, rnv, PF, a);
+        backward_coloc(m, s, gf, rsum * rnv, PF, a);
     }
     coloc_finish(a);
 #pragma unroll

@@ -497,11 +497,12 @@ def test_golden_trajectory_step_by_step(pkg, golden_dir, loss):
     assert lr == 0.01
     states, final = reference_trajectory(res, z["cam"].shape[0], size, z["cam"], z["light"], z["light_pow0"],
                                          z["init"], z["target"], 5, mode, "cpu")
-    # bit for bit where the golden file was written; within 2 ulp on a host
-    # whose CPU kernels take another vector path
+    # bit for bit where the golden file was written; on a host whose CPU
+    # kernels take another vector path, within 4 ulp of 1.0 (the maps lie in
+    # [-1, 1]; a relative bound is meaningless next to 0)
     for k in range(4):
-        np.testing.assert_array_max_ulp(states[k + 1][0], z["tex_textures"][k], maxulp=2)
-    np.testing.assert_array_max_ulp(final, z["tex_textures"][4], maxulp=2)
+        np.testing.assert_allclose(states[k + 1][0], z["tex_textures"][k], rtol=0, atol=2.4e-7)
+    np.testing.assert_allclose(final, z["tex_textures"][4], rtol=0, atol=2.4e-7)
     fs, _ = make_problem(pkg, z["init"], z["target"], res, size, z["cam"], z["light"], z["light_pow0"], loss)
     check_teacher_forced(pkg, fs, states, res, size, z["cam"], z["light"], z["light_pow0"], z["target"], mode,
                          "cpu", f"golden {loss}")
