@@ -69,47 +69,51 @@ __device__ __forceinline__ void source_coord(int d, int n_src, int n_dst, int &f
     first = (int)s - 3;
 }
 
-// Float weight of tap i for fraction x, before normalisation.
-__device__ __forceinline__ float lanczos4_tap(float x, int i) {
+// Float weights of the 8 taps for fraction x, before normalisation.  The
+// angle a0 = -(x + 3) pi / 4 is common to all taps (tap i's sine is a signed
+// combination of sin(a0) and cos(a0)), so one sin / cos pair serves a plan.
+__device__ __forceinline__ void lanczos4_taps(float x, float (&w)[8]) {
     constexpr double kS45 = 0.70710678118654752440;
     // (-1)^(i+1) (cos(i pi/4), sin(i pi/4))
     constexpr double kA[8] = {-1.0, kS45, 0.0, -kS45, 1.0, -kS45, 0.0, kS45};
     constexpr double kB[8] = {0.0, kS45, -1.0, kS45, 0.0, -kS45, 1.0, -kS45};
     const float xp3 = x + 3.0f;
-    const float t = xp3 - (float)i;
-    if (fabsf(t) < 1e-6f) return 1e30f;   // the unit impulse after normalisation
     const double a0 = -(double)xp3 * M_PI * 0.25;
-    const double y = -(double)t * M_PI * 0.25;
-    return (float)((kA[i] * sin(a0) + kB[i] * cos(a0)) / (y * y));
+    const double sa = sin(a0), ca = cos(a0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float t = xp3 - (float)i;
+        const double y = -(double)t * M_PI * 0.25;
+        // |t| < 1e-6: the unit impulse after normalisation
+        w[i] = fabsf(t) < 1e-6f ? 1e30f : (float)((kA[i] * sa + kB[i] * ca) / (y * y));
+    }
 }
 
-// The tile's 16 column and 16 row plans, one tap per thread (32 x 8 = 256):
-// float weights, then the sequential float sum of each plan's 8 weights and
-// its reciprocal, then the rounded 11-bit weights.
-__device__ void tile_plans(int x0, int y0, int sh, int sw, int dh, int dw, AxisPlan *plans, float (*wtmp)[8],
-                           float *winv) {
-    const int plan = threadIdx.x >> 3, i = threadIdx.x & 7;
-    const bool is_x = plan < kTile;
-    const int d = is_x ? x0 + plan : y0 + plan - kTile;
-    const bool live = d < (is_x ? dw : dh);
-    int first = 0;
-    float frac = 0.0f;
-    if (live) {
-        source_coord(d, is_x ? sw : sh, is_x ? dw : dh, first, frac);
-        wtmp[plan][i] = lanczos4_tap(frac, i);
-    }
-    __syncthreads();
-    if (live && i == 0) {
-        float total = 0.0f;
+// The tile's 16 column and 16 row plans, one plan per thread of the first 32:
+// float weights, their sequential float sum and its reciprocal, then the
+// rounded 11-bit weights.
+__device__ void tile_plans(int x0, int y0, int sh, int sw, int dh, int dw, AxisPlan *plans) {
+    const int plan = threadIdx.x;
+    if (plan < kPlans) {
+        const bool is_x = plan < kTile;
+        const int d = is_x ? x0 + plan : y0 + plan - kTile;
+        if (d < (is_x ? dw : dh)) {
+            int first = 0;
+            float frac = 0.0f;
+            source_coord(d, is_x ? sw : sh, is_x ? dw : dh, first, frac);
+            float w[8];
+            lanczos4_taps(frac, w);
+            float total = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) total = total + wtmp[plan][k];
-        winv[plan] = 1.0f / total;
-        plans[plan].first = first;
-    }
-    __syncthreads();
-    if (live) {
-        const float v = rintf((wtmp[plan][i] * winv[plan]) * (float)(1 << kCoefBits));
-        plans[plan].w[i] = (int)fminf(fmaxf(v, -32768.0f), 32767.0f);
+            for (int k = 0; k < 8; ++k) total = total + w[k];
+            const float inv = 1.0f / total;
+            plans[plan].first = first;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float v = rintf((w[i] * inv) * (float)(1 << kCoefBits));
+                plans[plan].w[i] = (int)fminf(fmaxf(v, -32768.0f), 32767.0f);
+            }
+        }
     }
     __syncthreads();
 }
@@ -128,11 +132,9 @@ template <int C>
 __global__ __launch_bounds__(kBlock) void resize_lanczos4_kernel(const uint8_t *__restrict__ src, int sh, int sw,
                                                                  uint8_t *__restrict__ dst, int dh, int dw) {
     __shared__ AxisPlan plans[kPlans];
-    __shared__ float wtmp[kPlans][8];
-    __shared__ float winv[kPlans];
     __shared__ int hsum[kMaxSpan][kTile][C];
     const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
-    tile_plans(x0, y0, sh, sw, dh, dw, plans, wtmp, winv);
+    tile_plans(x0, y0, sh, sw, dh, dw, plans);
     const AxisPlan *px = plans, *py = plans + kTile;
     const int rows_live = min(kTile, dh - y0), cols_live = min(kTile, dw - x0);
     const int r0 = py[0].first;

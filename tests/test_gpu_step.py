@@ -497,12 +497,12 @@ def test_golden_trajectory_step_by_step(pkg, golden_dir, loss):
     assert lr == 0.01
     states, final = reference_trajectory(res, z["cam"].shape[0], size, z["cam"], z["light"], z["light_pow0"],
                                          z["init"], z["target"], 5, mode, "cpu")
-    # bit for bit where the golden file was written; on a host whose CPU
-    # kernels take another vector path, within 4 ulp of 1.0 (the maps lie in
-    # [-1, 1]; a relative bound is meaningless next to 0)
-    for k in range(4):
-        np.testing.assert_allclose(states[k + 1][0], z["tex_textures"][k], rtol=0, atol=2.4e-7)
-    np.testing.assert_allclose(final, z["tex_textures"][4], rtol=0, atol=2.4e-7)
+    # bit for bit where the golden file was written (tests/test_oracle.py pins
+    # that); on a host whose CPU kernels take another vector path a few values
+    # differ in the last bits, which Adam carries on over the steps
+    for k, got in enumerate([st[0] for st in states[1:]] + [final]):
+        ref = z["tex_textures"][k]
+        assert np.mean(got == ref) >= 0.95 and np.abs(got - ref).max() <= 1e-6, (k, np.mean(got == ref))
     fs, _ = make_problem(pkg, z["init"], z["target"], res, size, z["cam"], z["light"], z["light_pow0"], loss)
     check_teacher_forced(pkg, fs, states, res, size, z["cam"], z["light"], z["light_pow0"], z["target"], mode,
                          "cpu", f"golden {loss}")
