@@ -66,6 +66,9 @@ sys.path.insert(0, str(ROOT))
 import __graft_entry__  # noqa: E402
 
 METRIC = "per-pixel optim iters/sec at 1024²×9 views (fused vs torch autograd); HBM GB/s"
+# synthetic materials of every workload: one albedo / specular / roughness draw
+# per channel for the whole map, detail in the height-field normals
+ALBEDO = "material"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -123,7 +126,7 @@ def algorithmic_bytes_adjoint(res, views):
 
 def make_problem(pkg, res, views, dev, loss="log_l2"):
     syn = pkg.synthetic
-    gt = syn.ground_truth_textures(res, seed=0)
+    gt = syn.ground_truth_textures(res, seed=0, albedo=ALBEDO)
     init = syn.init_textures(gt, seed=1)
     cam, light = syn.flash_grid(n_views=views, jitter=3.0 if views != 9 else 0.0,
                                 rng=np.random.default_rng(3))
@@ -285,7 +288,7 @@ def batch_materials(pkg, dev, seed=3):
     views = rng.integers(4, 17, len(sizes))
     mats = []
     for i, (res, V) in enumerate(zip(sizes, views)):
-        gt = syn.ground_truth_textures(res, seed=1000 + i)
+        gt = syn.ground_truth_textures(res, seed=1000 + i, albedo=ALBEDO)
         cam, light = syn.flash_grid(jitter=3.0, rng=rng, n_views=int(V))
         mats.append((res, int(V), gt, syn.init_textures(gt, seed=2000 + i), cam, light))
     return mats
@@ -373,7 +376,7 @@ def staged_leg(pkg, dev, views, stages=(256, 512, 1024), iters=20):
     total = 0.0
     # one material: the finest ground truth, box-averaged for the coarse stages
     finest = max(stages)
-    gt_fine = syn.ground_truth_textures(finest, seed=0)
+    gt_fine = syn.ground_truth_textures(finest, seed=0, albedo=ALBEDO)
     for res in stages:
         cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
         r = pkg.Microfacet(res, views, syn.PLANE_SIZE_CM, cl, dev)
@@ -474,7 +477,7 @@ def driver_leg(pkg, dev, res=1024, epochs=1000):
                                    "camera_pos": cam.tolist(), "light_pos": light.tolist(),
                                    "light_pow": syn.light_power().tolist()}))
         sio = pkg.SvbrdfIO(cfg, dev)
-        sio.save_textures_th(torch.tensor(syn.ground_truth_textures(res, seed=0), device=dev)[None],
+        sio.save_textures_th(torch.tensor(syn.ground_truth_textures(res, seed=0, albedo=ALBEDO), device=dev)[None],
                              sio.reference_dir)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -600,7 +603,7 @@ def compose_leg(pkg, dev, res=256, views=9, iters=100):
     5 layers + pixel L2 -> backward (HIP adjoint) -> torch Adam, 100 steps."""
     from svbrdf_amd import compose
     syn = pkg.synthetic
-    gt = syn.ground_truth_textures(res, seed=0)
+    gt = syn.ground_truth_textures(res, seed=0, albedo=ALBEDO)
     cam, light = syn.flash_grid(n_views=views)
     cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(syn.light_power())]
     r = pkg.Microfacet(res, views, syn.PLANE_SIZE_CM, cl, dev)
@@ -671,7 +674,7 @@ def cpu_forward(res, views, n=3):
     cam, light = syn.flash_grid(n_views=views)
     cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(syn.light_power())]
     r = MT.MicrofacetEager(res, views, syn.PLANE_SIZE_CM, cl, "cpu")
-    gt = torch.tensor(syn.ground_truth_textures(res, seed=0))[None]
+    gt = torch.tensor(syn.ground_truth_textures(res, seed=0, albedo=ALBEDO))[None]
     times = []
     with torch.no_grad():
         for _ in range(n):
@@ -692,7 +695,7 @@ def cpu_baseline(res, views, seconds):
     syn = pkg.synthetic
     threads, host = host_cpu()
     torch.set_num_threads(threads)
-    gt = syn.ground_truth_textures(res, seed=0)
+    gt = syn.ground_truth_textures(res, seed=0, albedo=ALBEDO)
     init = syn.init_textures(gt, seed=1)
     cam, light = syn.flash_grid(n_views=views)
     P = syn.light_power()

@@ -8,9 +8,15 @@ every workload here is generated.  The recipe:
   that max|h| = amplitude (0.5); n = normalize(-dh/dx, -dh/dy, 1) with
   central differences in pixel units; (nx, ny) go to channels 3:5 as in
   src/svbrdf.py:160,166 (normals are stored directly, not as 2u-1).
-* diffuse u ~ U[0.05, 0.9] per pixel per RGB, specular u ~ U[0.02, 0.3]
-  per pixel per RGB, roughness u ~ U[0.1, 0.9] per pixel; stored as
-  t = 2u - 1 (src/svbrdf.py:161-163).
+* diffuse u ~ U[0.05, 0.9] per RGB, specular u ~ U[0.02, 0.3] per RGB,
+  roughness u ~ U[0.1, 0.9]; stored as t = 2u - 1 (src/svbrdf.py:161-163).
+  Two readings of "per RGB": ``albedo="texel"`` draws every texel
+  independently (each map covers its whole value range in one image, which
+  the parity tests want), ``albedo="material"`` draws one value per channel
+  for the whole map (a uniformly coloured material whose spatial detail is
+  the smooth normal field; a coarser grid represents it, which
+  coarse-to-fine stage chaining presupposes).  The benchmark workloads use
+  "material".
 * views: a 3x3 grid (x, y) in {-8, 0, 8}^2 cm at z = 30 cm, +y row first
   and x from - to + within a row (the order of data/random/render.json:7-24),
   light co-located with the camera, light power [1500, 1500, 1500];
@@ -49,13 +55,18 @@ def height_field_normals(res, rng, sigma_px=None, amplitude=0.5):
     return n[0].astype(np.float32), n[1].astype(np.float32)
 
 
-def ground_truth_textures(res, seed=0, sigma_px=None):
-    """``[9, R, R]`` float32 maps in [-1, 1]: dif 0:3, normal 3:5, rgh 5, spe 6:9."""
+def ground_truth_textures(res, seed=0, sigma_px=None, albedo="texel"):
+    """``[9, R, R]`` float32 maps in [-1, 1]: dif 0:3, normal 3:5, rgh 5, spe 6:9.
+    albedo = "texel" (an independent draw per texel) or "material" (one draw
+    per channel for the whole map)."""
+    if albedo not in ("texel", "material"):
+        raise ValueError(f"albedo must be 'texel' or 'material', got {albedo!r}")
     rng = np.random.Generator(np.random.PCG64(seed))
     nx, ny = height_field_normals(res, rng, sigma_px)
-    dif = rng.uniform(0.05, 0.9, (3, res, res))
-    spe = rng.uniform(0.02, 0.3, (3, res, res))
-    rgh = rng.uniform(0.1, 0.9, (1, res, res))
+    n = (res, res) if albedo == "texel" else (1, 1)
+    dif = rng.uniform(0.05, 0.9, (3, *n))
+    spe = rng.uniform(0.02, 0.3, (3, *n))
+    rgh = rng.uniform(0.1, 0.9, (1, *n))
     t = np.empty((9, res, res), np.float32)
     t[0:3] = dif * 2 - 1
     t[3] = nx

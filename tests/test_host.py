@@ -218,3 +218,20 @@ def test_init_from_randn_statistics(pkg):
     o2 = _optim_on_host(pkg, R)
     o2.init_from_randn()
     assert torch.equal(o.textures, o2.textures)
+
+
+def test_material_recipe_is_uniform_per_channel(pkg):
+    """albedo="material": one draw per channel for the whole map (diffuse,
+    specular, roughness constant over the image, in their ranges); the
+    normals are the per-texel recipe's, from the same seed."""
+    syn = pkg.synthetic
+    m = syn.ground_truth_textures(64, seed=4, albedo="material")
+    t = syn.ground_truth_textures(64, seed=4)
+    for c in (0, 1, 2, 5, 6, 7, 8):
+        assert np.unique(m[c]).size == 1, c
+    u = (m[:, 0, 0].astype(np.float64) + 1) / 2
+    assert np.all((0.05 <= u[0:3]) & (u[0:3] <= 0.9)) and np.all((0.02 <= u[6:9]) & (u[6:9] <= 0.3))
+    assert 0.1 <= u[5] <= 0.9
+    np.testing.assert_array_equal(m[3:5], t[3:5])
+    with pytest.raises(ValueError):
+        syn.ground_truth_textures(8, albedo="pixel")
