@@ -101,6 +101,10 @@ class MicrofacetEager:
         # src/microfacet.py:84-120
         assert textures.shape[2] == textures.shape[3]
         assert self.res == textures.shape[2]
+        return self.radiance(textures).clamp(self.eps, 1) ** (1 / 2.2)
+
+    def radiance(self, textures):
+        # src/microfacet.py:86-117: the image before the output clamp and gamma
         nrm, dif, spe, rgh = self._maps(textures)
         v, _ = self._get_dir(self.camera_pos)
         l, d2 = self._get_dir(self.light_pos)
@@ -116,8 +120,7 @@ class MicrofacetEager:
         G = self._smith(nv, nl, rgh ** 2)
         f2 = D * F * G / (4 * nv * nl + self.eps)
         f = f1 + f2
-        img = self.light_pow * f * nl / d2
-        return img.clamp(self.eps, 1) ** (1 / 2.2)
+        return self.light_pow * f * nl / d2
 
 
 def target_floor():

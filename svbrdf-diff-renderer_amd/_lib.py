@@ -10,7 +10,9 @@ import pathlib
 import torch
 
 _PKG = pathlib.Path(__file__).resolve().parent
-LIB_PATH = _PKG / "libsvbrdf_hip.so"
+# SVBRDF_AMD_LIB selects another build of the same library (the host-side
+# sanitizer build of csrc/build.py --sanitize, for the CPU tests)
+LIB_PATH = pathlib.Path(os.environ.get("SVBRDF_AMD_LIB") or _PKG / "libsvbrdf_hip.so")
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
 ABI_VERSION = 4

@@ -16,6 +16,12 @@ Gradients (FP32), per texture channel:
   relative of the FP64 gradient, or no worse than twice the error of the
   reference's own FP32 autograd gradient where FP32 cannot reach 1e-4; and
   ||g_gpu - g_ref32|| <= 1e-4 ||g_ref32|| + 3 ||g_ref32 - g64||.
+  Pixels at a kink of the output clamp may be left out of the norms: a
+  pixel one of whose pre-clamp image values (FP64) lies within 2e-6
+  relative of a clamp bound (1e-6 or 1).  There rounding decides whether a
+  whole view's contribution passes the clamp's mask, in any FP32
+  implementation (the reference's included), and the gradient jumps.  At
+  most 1e-4 of the pixels may be left out.
 """
 import numpy as np
 
@@ -42,8 +48,25 @@ def check_forward(out, ref32, ref64, sigma, label=""):
                 frac_plain_vs_fp64=frac64)
 
 
-def check_gradient(g, g32, g64, label=""):
+KINK_WINDOW = 2e-6
+MAX_KINK_FRACTION = 1e-4
+
+
+def clamp_kinks(img64, eps=1e-6):
+    """[R, R] mask of the pixels whose pre-clamp image ([V, 3, R, R], FP64)
+    lies within KINK_WINDOW relative of a clamp bound in some view and
+    channel."""
+    img64 = np.asarray(img64)
+    near = (np.abs(img64 / eps - 1.0) <= KINK_WINDOW) | (np.abs(img64 - 1.0) <= KINK_WINDOW)
+    return near.any(axis=(0, 1))
+
+
+def check_gradient(g, g32, g64, label="", exclude=None):
     g = np.asarray(g, np.float64).reshape(g64.shape)
+    if exclude is not None and exclude.any():
+        assert exclude.mean() <= MAX_KINK_FRACTION, f"{label}: {int(exclude.sum())} kink pixels"
+        keep = ~exclude
+        g, g32, g64 = g * keep, np.asarray(g32) * keep, np.asarray(g64) * keep
     out = []
     for c in range(g64.shape[0]):
         n64 = np.linalg.norm(g64[c])
