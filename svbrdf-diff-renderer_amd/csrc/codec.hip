@@ -15,7 +15,8 @@
 //   textures_to_u8_kernel   [9, R, R] f32 -> 4 maps u8             36 B read, 10 B written / px
 //   images_to_u8_kernel     [n, C, h, w] f32 -> [n, h, w, C] u8    4 C B read, C B written / px
 //   upsample2x_kernel       [P, R, R] f32 -> [P, 2R, 2R] f32 bilinear (stage chaining in memory);
-//   upsample2x_x4_kernel    the same, 2 x 4 outputs per thread (even R, aligned destination rows)
+//   upsample2x_x4_kernel    the same, 2 x 4 outputs per thread and plane, several planes per thread
+//                           (even R, aligned rows)
 //
 // Float arithmetic follows the reference's float32 numpy / torch op order one
 // rounding per op: contraction into fused multiply-adds is off in this file,
@@ -520,47 +521,69 @@ __global__ __launch_bounds__(kBlock) void upsample2x_kernel(const float *__restr
 // and scale 2 the weights are 1/4 and 3/4 everywhere except output row /
 // column 0 (weights 1 and 0) and the neighbour index is clamped at the last
 // row / column, so a thread needs source rows i - 1, i, i + 1 and columns
-// 2q - 1 .. 2q + 2 (clamped): 12 loads and two float4 stores per plane
-// instead of 32 loads and eight scalar stores.  Every output is the same
-// expression, in the same order, as upsample2x_kernel evaluates it
-// (h0 (w0 a + w1 b) + h1 (w0 c + w1 d), one rounding per operation), so the
-// two kernels agree bit for bit.  blockIdx.y is the plane.
+// 2q - 1 .. 2q + 2 (clamped): per plane three float2 and six scalar loads
+// and two float4 stores, instead of 32 loads and eight scalar stores.  A
+// thread takes kUpPlanes planes (blockIdx.y selects the group) and issues all
+// their loads before the first store, so that each wave has that many
+// loads in flight.  Every output is the same expression, in the same order,
+// as upsample2x_kernel evaluates it (h0 (w0 a + w1 b) + h1 (w0 c + w1 d), one
+// rounding per operation), so the two kernels agree bit for bit.
+#ifndef SVBRDF_UP_PLANES
+#define SVBRDF_UP_PLANES 3
+#endif
+constexpr int kUpPlanes = SVBRDF_UP_PLANES;
+
 __global__ __launch_bounds__(kBlock) void upsample2x_x4_kernel(const float *__restrict__ src, unsigned res,
                                                                size_t src_stride, float *__restrict__ dst,
-                                                               size_t dst_stride) {
+                                                               size_t dst_stride, int n_planes) {
     const unsigned half = res / 2u;
     const unsigned t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= res * half) return;
     const unsigned i = t / half, q = t - i * half;
-    const float *s = src + (size_t)blockIdx.y * src_stride;
-    float *d = dst + (size_t)blockIdx.y * dst_stride;
     const unsigned last = res - 1u;
-    const unsigned c0 = q == 0u ? 0u : 2u * q - 1u, c1 = 2u * q, c2 = 2u * q + 1u,
-                   c3 = 2u * q + 2u > last ? last : 2u * q + 2u;
+    const unsigned c0 = q == 0u ? 0u : 2u * q - 1u, c1 = 2u * q, c3 = 2u * q + 2u > last ? last : 2u * q + 2u;
     const bool left = q == 0u, top = i == 0u;
-    // horizontal pass of one source row: output columns 4q .. 4q + 3
-    auto hrow = [&](unsigned r, float (&h)[4]) {
-        const float *row = s + (size_t)r * res;
-        const float w0 = row[c0], w1 = row[c1], w2 = row[c2], w3 = row[c3];
-        h[0] = left ? 1.0f * w1 + 0.0f * w2 : 0.25f * w0 + 0.75f * w1;
-        h[1] = 0.75f * w1 + 0.25f * w2;
-        h[2] = 0.25f * w1 + 0.75f * w2;
-        h[3] = 0.75f * w2 + 0.25f * w3;
-    };
-    const unsigned rm = top ? 0u : i - 1u, rp = i + 1u > last ? last : i + 1u;
-    float hm[4], hi[4], hp[4];
-    hrow(rm, hm);
-    hrow(i, hi);
-    hrow(rp, hp);
-    float a[4], b[4];
+    const unsigned rows[3] = {top ? 0u : i - 1u, i, i + 1u > last ? last : i + 1u};
+    const int p0 = (int)blockIdx.y * kUpPlanes;
+    // every load of the thread's planes first: win[plane][row][column 2q-1 .. 2q+2]
+    float win[kUpPlanes][3][4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        a[k] = top ? 1.0f * hi[k] + 0.0f * hp[k] : 0.25f * hm[k] + 0.75f * hi[k];
-        b[k] = 0.75f * hi[k] + 0.25f * hp[k];
+    for (int pl = 0; pl < kUpPlanes; ++pl) {
+        const int c = p0 + pl < n_planes ? p0 + pl : n_planes - 1;   // a tail group re-reads the last plane
+        const float *s = src + (size_t)c * src_stride;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float *row = s + (size_t)rows[r] * res;
+            const float2 mid = *reinterpret_cast<const float2 *>(row + c1);
+            win[pl][r][0] = row[c0];
+            win[pl][r][1] = mid.x;
+            win[pl][r][2] = mid.y;
+            win[pl][r][3] = row[c3];
+        }
     }
-    const size_t o = (size_t)(2u * i) * (2u * res) + 4u * q;
-    *reinterpret_cast<float4 *>(d + o) = make_float4(a[0], a[1], a[2], a[3]);
-    *reinterpret_cast<float4 *>(d + o + 2u * res) = make_float4(b[0], b[1], b[2], b[3]);
+#pragma unroll
+    for (int pl = 0; pl < kUpPlanes; ++pl) {
+        if (p0 + pl >= n_planes) break;
+        float h[3][4];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float w0 = win[pl][r][0], w1 = win[pl][r][1], w2 = win[pl][r][2], w3 = win[pl][r][3];
+            h[r][0] = left ? 1.0f * w1 + 0.0f * w2 : 0.25f * w0 + 0.75f * w1;
+            h[r][1] = 0.75f * w1 + 0.25f * w2;
+            h[r][2] = 0.25f * w1 + 0.75f * w2;
+            h[r][3] = 0.75f * w2 + 0.25f * w3;
+        }
+        float a[4], b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            a[k] = top ? 1.0f * h[1][k] + 0.0f * h[2][k] : 0.25f * h[0][k] + 0.75f * h[1][k];
+            b[k] = 0.75f * h[1][k] + 0.25f * h[2][k];
+        }
+        float *d = dst + (size_t)(p0 + pl) * dst_stride;
+        const size_t o = (size_t)(2u * i) * (2u * res) + 4u * q;
+        *reinterpret_cast<float4 *>(d + o) = make_float4(a[0], a[1], a[2], a[3]);
+        *reinterpret_cast<float4 *>(d + o + 2u * res) = make_float4(b[0], b[1], b[2], b[3]);
+    }
 }
 
 bool aligned(const void *p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
@@ -629,10 +652,11 @@ int svbrdf_upsample2x_bilinear(const float *src, int res, size_t src_stride, flo
     const uintptr_t d0 = (uintptr_t)dst, d1 = d0 + ((n_planes - 1) * dst_stride + n_out) * sizeof(float);
     if (s0 < d1 && d0 < s1) return fail(SVBRDF_EINVAL, "%s: source and destination overlap", what);
     hipStream_t s = (hipStream_t)stream;
-    if (res % 2 == 0 && dst_stride % 4 == 0 && aligned(dst, 16)) {
+    if (res % 2 == 0 && dst_stride % 4 == 0 && aligned(dst, 16) && src_stride % 2 == 0 && aligned(src, 8)) {
         const unsigned threads = (unsigned)res * (unsigned)(res / 2);
-        hipLaunchKernelGGL(upsample2x_x4_kernel, dim3(blocks_for(threads), (unsigned)n_planes), dim3(kBlock), 0, s,
-                           src, (unsigned)res, src_stride, dst, dst_stride);
+        const unsigned groups = (unsigned)((n_planes + kUpPlanes - 1) / kUpPlanes);
+        hipLaunchKernelGGL(upsample2x_x4_kernel, dim3(blocks_for(threads), groups), dim3(kBlock), 0, s, src,
+                           (unsigned)res, src_stride, dst, dst_stride, n_planes);
     } else {
         hipLaunchKernelGGL(upsample2x_kernel, dim3(blocks_for(n_out)), dim3(kBlock), 0, s, src, (unsigned)res,
                            src_stride, dst, dst_stride, n_planes);
