@@ -338,6 +338,51 @@ int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const u
                       float *loss_history, int history_len, void *workspace, size_t workspace_bytes,
                       void *stream);
 
+/* ------------------------------------------------------------------------
+ * Convolutions of the MaterialGAN caller's networks (src/materialgan.py:21-173)
+ * on the matrix cores, fp32 as the reference computes them: the 3x3
+ * modulated / demodulated convolutions of the StyleGAN2 synthesis network
+ * (ModulateConvBlock, src/higan_models/stylegan2_generator_network.py:585-760)
+ * and the 3x3 convolutions of the VGG-19 feature loss (src/descriptor.py:7-79).
+ * ------------------------------------------------------------------------ */
+
+enum svbrdf_activation {
+    SVBRDF_ACT_NONE = 0,
+    SVBRDF_ACT_RELU = 1,    /* max(v, 0): VGG (src/descriptor.py:7-79)                              */
+    SVBRDF_ACT_LRELU = 2,   /* leaky ReLU (slope 0.2) x sqrt(2): ModulateConvBlock's activation     */
+};
+
+/* Weights [cout, cin, 3, 3] -> the layout svbrdf_conv3x3 reads, [9][cin][cout]
+ * (tap = 3 kh + kw), each value scale * style[ci] * w (style may be NULL: 1)
+ * and, when demodulate is non-zero, divided per output channel by
+ * sqrt(sum over ci, taps of that value^2 + 1e-8) (the style modulation and
+ * demodulation of ModulateConvBlock.forward, stylegan2_generator_network.py:694-706).
+ * transpose != 0 writes the input-gradient weights instead: [9][cout][cin]
+ * with the taps flipped (8 - tap), so that svbrdf_conv3x3 of dL/dy with them
+ * (cin and cout exchanged) is dL/dx. */
+int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *style, float scale, int demodulate,
+                        int transpose, float *packed, void *stream);
+
+/* y = act(conv3x3(x, w) + noise_strength * noise + bias), stride 1, zero
+ * padding 1 (F.conv2d(x, w, padding=1) as the reference's conv layers call
+ * it): x [n, cin, height, width], packed from svbrdf_conv3x3_pack, bias
+ * [cout] or NULL, noise [height, width] (shared by the n images) or NULL, y
+ * [n, cout, height, width].  The noise and bias terms and the activation are
+ * ModulateConvBlock's (stylegan2_generator_network.py:738-755) or VGG's
+ * (bias, ReLU). */
+int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const float *packed, int cout,
+                   const float *bias, const float *noise, float noise_strength, int activation, float *y,
+                   void *stream);
+
+/* x2 upsampling of `planes` planes of height x width: zero insertion and the
+ * [1, 3, 3, 1] binomial filter with gain 4 (UpsamplingLayer,
+ * stylegan2_generator_network.py:419-479, as the synthesis network's x2
+ * resolution steps use it), y [planes, 2 height, 2 width]; and its adjoint
+ * (grad_y [planes, 2 height, 2 width] -> grad_x [planes, height, width]). */
+int svbrdf_upsample2_blur(const float *x, int planes, int height, int width, float *y, void *stream);
+int svbrdf_upsample2_blur_adjoint(const float *grad_y, int planes, int height, int width, float *grad_x,
+                                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

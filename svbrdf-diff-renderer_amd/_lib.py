@@ -27,6 +27,7 @@ LOSS_L2 = 0
 LOSS_LOG_L2 = 1
 IMREAD_NONE, IMREAD_SRGB, IMREAD_ROUGH, IMREAD_NORMAL = 0, 1, 2, 3
 TERMS_EVAL, NDF_BECKMANN, FRESNEL_SCHLICK = 0, 1, 2
+ACT_NONE, ACT_RELU, ACT_LRELU = 0, 1, 2
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
@@ -78,6 +79,13 @@ SIGNATURES = {
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp,
                                          ctypes.c_int, _vp, ctypes.c_size_t, _vp]),
+    # convolutions of the MaterialGAN caller's networks
+    "svbrdf_conv3x3_pack": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_float, ctypes.c_int,
+                                           ctypes.c_int, _vp, _vp]),
+    "svbrdf_conv3x3": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                                      ctypes.c_int, _vp, _vp, ctypes.c_float, ctypes.c_int, _vp, _vp]),
+    "svbrdf_upsample2_blur": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "svbrdf_upsample2_blur_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
 }
 
 _lib = None

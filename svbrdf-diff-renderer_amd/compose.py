@@ -9,10 +9,13 @@ builds a VGG-19-SHAPED stack with seeded He-normal weights: conv1_1 ...
 conv5_1 (3x3 convolutions, ReLU, 2x2 average pooling between stages).
 The loss is the L2 distance of Gram matrices at relu1_1, relu2_1,
 relu3_1, relu4_1, relu5_1 plus a pixel L2 term.  The renderer's adjoint
-kernel receives dL/dimages from torch's convolution backward through
-:class:`svbrdf_amd.RenderFunction`.
+kernel receives dL/dimages from the feature stack's backward (the
+convolutions' input gradients are the matrix-core kernel of
+:mod:`svbrdf_amd.convnet`) through :class:`svbrdf_amd.RenderFunction`.
 """
 import torch
+
+from . import convnet
 
 # (out_channels) per conv layer, 'P' = 2x2 average pool; up to conv5_1
 VGG19_TO_CONV5_1 = [64, 64, "P", 128, 128, "P", 256, 256, 256, 256, "P", 512, 512, 512, 512, "P", 512]
@@ -20,29 +23,24 @@ STYLE_LAYERS = (0, 2, 4, 8, 12)   # indices of conv1_1, conv2_1, conv3_1, conv4_
 
 
 class VGGShapedFeatures(torch.nn.Module):
-    """Random-init (He-normal, seeded) VGG-19-shaped conv stack up to conv5_1."""
+    """Random-init (He-normal, seeded) VGG-19-shaped conv stack up to conv5_1;
+    each 3x3 convolution with its ReLU is a :class:`convnet.Conv3x3` layer
+    (the matrix-core kernel, forward and input gradient)."""
 
-    def __init__(self, seed=4, device="cuda", dtype=torch.float32):
+    def __init__(self, seed=4, device="cuda"):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
-        self.convs = torch.nn.ModuleList()
-        self.plan = []
+        self.convs, self.plan = [], []
         cin = 3
         for item in VGG19_TO_CONV5_1:
             if item == "P":
                 self.plan.append("P")
                 continue
-            conv = torch.nn.Conv2d(cin, item, 3, padding=1)
-            with torch.no_grad():
-                std = (2.0 / (cin * 9)) ** 0.5
-                conv.weight.copy_(torch.randn(conv.weight.shape, generator=g) * std)
-                conv.bias.zero_()
-            self.convs.append(conv)
+            std = (2.0 / (cin * 9)) ** 0.5
+            wgt = torch.randn((item, cin, 3, 3), generator=g) * std
+            self.convs.append(convnet.Conv3x3(wgt.to(device), torch.zeros(item, device=device), act="relu"))
             self.plan.append(len(self.convs) - 1)
             cin = item
-        self.to(device=device, dtype=dtype)
-        for p in self.parameters():
-            p.requires_grad_(False)
 
     def forward(self, x):
         feats = []
@@ -50,7 +48,7 @@ class VGGShapedFeatures(torch.nn.Module):
             if step == "P":
                 x = torch.nn.functional.avg_pool2d(x, 2)
             else:
-                x = torch.relu(self.convs[step](x))
+                x = self.convs[step](x)
                 if step in STYLE_LAYERS:
                     feats.append(x)
         return feats

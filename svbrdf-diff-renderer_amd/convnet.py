@@ -1,0 +1,169 @@
+"""Convolution layers of the MaterialGAN caller's networks on the matrix-core
+kernels of libsvbrdf_hip.so (csrc/conv.hip).
+
+* :func:`conv3x3` -- a 3x3 convolution (stride 1, zero padding 1) with the
+  fused epilogue of the reference's blocks: ``act(conv + noise_strength *
+  noise + bias)``, act = identity, ReLU (VGG, src/descriptor.py:7-79) or
+  leaky ReLU x sqrt(2) (ModulateConvBlock,
+  src/higan_models/stylegan2_generator_network.py:738-755).  Differentiable
+  with respect to the input, the weights and the noise map: the forward and
+  dL/dx are the HIP kernel (dL/dx is the convolution of dL/dy with the
+  flipped, transposed weights), dL/dweights is torch's weight-gradient
+  convolution.
+* :func:`upsample2` -- x2 upsampling by zero insertion and the [1, 3, 3, 1]
+  filter (UpsamplingLayer, stylegan2_generator_network.py:419-479); forward
+  and adjoint are HIP kernels.
+* :class:`Conv3x3` -- a frozen 3x3 layer (VGG): weights packed once, in both
+  layouts.
+"""
+import torch
+
+from . import _lib
+
+ACT = {"none": _lib.ACT_NONE, "relu": _lib.ACT_RELU, "lrelu": _lib.ACT_LRELU}
+
+
+def _check(t, name):
+    if t.device.type != "cuda" or t.dtype != torch.float32:
+        raise ValueError(f"{name} must be a float32 CUDA tensor (there is no CPU path), got {t.dtype} on {t.device}")
+
+
+def pack3x3(weight, style=None, scale=1.0, demodulate=False, transpose=False):
+    """[Cout, Cin, 3, 3] -> the kernel's layout ([9, Cin, Cout], or the input-
+    gradient layout [9, Cout, Cin] with flipped taps when transpose), times
+    scale * style[ci] and demodulated per output channel when asked."""
+    w = weight.detach().contiguous()
+    _check(w, "weight")
+    cout, cin = int(w.shape[0]), int(w.shape[1])
+    if tuple(w.shape[2:]) != (3, 3):
+        raise ValueError(f"weight must be [Cout, Cin, 3, 3], got {tuple(w.shape)}")
+    st = None
+    if style is not None:
+        st = style.detach().reshape(-1).contiguous()
+        _check(st, "style")
+        if st.numel() != cin:
+            raise ValueError(f"style must have {cin} values, got {st.numel()}")
+    out = torch.empty((9, cout, cin) if transpose else (9, cin, cout), dtype=torch.float32, device=w.device)
+    _lib.call("svbrdf_conv3x3_pack", _lib.ptr(w), cout, cin, _lib.ptr(st), float(scale), int(bool(demodulate)),
+              int(bool(transpose)), _lib.ptr(out), _lib.stream_handle(w.device))
+    return out
+
+
+def conv3x3_packed(x, packed, cout, bias=None, noise=None, noise_strength=0.0, act="none"):
+    """The kernel itself on packed weights: x [N, Cin, H, W] -> [N, Cout, H, W]."""
+    x = x.contiguous()
+    _check(x, "x")
+    n, cin, h, w = (int(s) for s in x.shape)
+    if tuple(packed.shape) != (9, cin, cout):
+        raise ValueError(f"packed weights {tuple(packed.shape)} do not match Cin={cin}, Cout={cout}")
+    if bias is not None:
+        bias = bias.detach().reshape(-1).contiguous()
+        _check(bias, "bias")
+    if noise is not None:
+        noise = noise.detach().reshape(-1).contiguous()
+        _check(noise, "noise")
+        if noise.numel() != h * w:
+            raise ValueError(f"noise must have {h * w} values, got {noise.numel()}")
+    y = torch.empty((n, cout, h, w), dtype=torch.float32, device=x.device)
+    _lib.call("svbrdf_conv3x3", _lib.ptr(x), n, cin, h, w, _lib.ptr(packed), cout, _lib.ptr(bias), _lib.ptr(noise),
+              float(noise_strength), ACT[act], _lib.ptr(y), _lib.stream_handle(x.device))
+    return y
+
+
+def _act_grad(grad_y, y, act):
+    """dL/d(pre-activation) from dL/dy and y: the activations keep the sign."""
+    if act == "relu":
+        return grad_y * (y > 0)
+    if act == "lrelu":
+        return grad_y * torch.where(y > 0, 1.4142135623730951, 0.2 * 1.4142135623730951)
+    return grad_y
+
+
+class Conv3x3Function(torch.autograd.Function):
+    """y = act(conv3x3(x, weight) + noise_strength * noise + bias)."""
+
+    @staticmethod
+    def forward(ctx, x, weight, bias, noise, noise_strength, act):
+        packed = pack3x3(weight)
+        y = conv3x3_packed(x, packed, int(weight.shape[0]), bias, noise, noise_strength, act)
+        ctx.save_for_backward(x, weight, noise, y)
+        ctx.noise_strength, ctx.act = float(noise_strength), act
+        return y
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        x, weight, noise, y = ctx.saved_tensors
+        gz = _act_grad(grad_y.contiguous(), y, ctx.act).contiguous()
+        gx = gw = gnoise = None
+        if ctx.needs_input_grad[0]:
+            gx = conv3x3_packed(gz, pack3x3(weight, transpose=True), int(weight.shape[1]))
+        if ctx.needs_input_grad[1]:
+            gw = torch.nn.grad.conv2d_weight(x, weight.shape, gz, padding=1)
+        if ctx.needs_input_grad[3] and noise is not None:
+            gnoise = (ctx.noise_strength * gz.sum(dim=(0, 1))).reshape(noise.shape)
+        return gx, gw, None, gnoise, None, None
+
+
+def conv3x3(x, weight, bias=None, noise=None, noise_strength=0.0, act="none"):
+    if act not in ACT:
+        raise ValueError(f"act must be one of {sorted(ACT)}")
+    return Conv3x3Function.apply(x, weight, bias, noise, noise_strength, act)
+
+
+class Upsample2Function(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        x = x.contiguous()
+        _check(x, "x")
+        n, c, h, w = (int(s) for s in x.shape)
+        y = torch.empty((n, c, 2 * h, 2 * w), dtype=torch.float32, device=x.device)
+        _lib.call("svbrdf_upsample2_blur", _lib.ptr(x), n * c, h, w, _lib.ptr(y), _lib.stream_handle(x.device))
+        return y
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        g = grad_y.contiguous()
+        n, c, h2, w2 = (int(s) for s in g.shape)
+        gx = torch.empty((n, c, h2 // 2, w2 // 2), dtype=torch.float32, device=g.device)
+        _lib.call("svbrdf_upsample2_blur_adjoint", _lib.ptr(g), n * c, h2 // 2, w2 // 2, _lib.ptr(gx),
+                  _lib.stream_handle(g.device))
+        return gx
+
+
+def upsample2(x):
+    return Upsample2Function.apply(x)
+
+
+class _FrozenConvFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, layer):
+        y = conv3x3_packed(x, layer.packed, layer.cout, layer.bias, None, 0.0, layer.act)
+        ctx.save_for_backward(y)
+        ctx.layer = layer
+        return y
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        (y,) = ctx.saved_tensors
+        layer = ctx.layer
+        gz = _act_grad(grad_y.contiguous(), y, layer.act).contiguous()
+        return conv3x3_packed(gz, layer.packed_t, layer.cin), None
+
+
+class Conv3x3(torch.nn.Module):
+    """A 3x3 layer with frozen weights [Cout, Cin, 3, 3] and bias [Cout],
+    followed by `act`: packed once in the forward and input-gradient layouts."""
+
+    def __init__(self, weight, bias=None, act="relu"):
+        super().__init__()
+        if act not in ACT:
+            raise ValueError(f"act must be one of {sorted(ACT)}")
+        self.cout, self.cin = int(weight.shape[0]), int(weight.shape[1])
+        self.weight = weight.detach().contiguous()
+        self.bias = None if bias is None else bias.detach().contiguous()
+        self.act = act
+        self.packed = pack3x3(self.weight)
+        self.packed_t = pack3x3(self.weight, transpose=True)
+
+    def forward(self, x):
+        return _FrozenConvFunction.apply(x, self)

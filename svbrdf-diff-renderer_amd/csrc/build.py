@@ -16,7 +16,7 @@ CSRC = pathlib.Path(__file__).resolve().parent
 PKG = CSRC.parent
 ROOT = PKG.parent
 LIB = PKG / "libsvbrdf_hip.so"
-SOURCES = [CSRC / "svbrdf_hip.hip", CSRC / "codec.hip", CSRC / "common.hip"]
+SOURCES = [CSRC / "svbrdf_hip.hip", CSRC / "codec.hip", CSRC / "conv.hip", CSRC / "common.hip"]
 HEADERS = [CSRC / "brdf.hpp", CSRC / "common.hpp", ROOT / "include" / "svbrdf_hip.h"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -27,8 +27,13 @@ weights:
 * :class:`MaterialGANOptim` -- init_from, load_targets, compute_feature_loss,
   latent_to_textures (clamps of src/materialgan.py:91-100) and optim.
 
-The generator and VGG run as torch convolutions; the renderer is
-:class:`svbrdf_amd.Microfacet` (forward and adjoint kernels).
+The 3x3 convolutions of both networks run on the matrix-core kernels of
+:mod:`svbrdf_amd.convnet` (forward and input gradient; the generator's
+weight gradient, which the latent's gradient needs, is torch's), with the
+noise, bias and leaky-ReLU epilogue fused; the x2 upsampling is a HIP kernel
+too.  The renderer is :class:`svbrdf_amd.Microfacet` (forward and adjoint
+kernels).  The 1x1 toRGB convolutions (9 output channels) and the
+fully connected mapping layers stay torch operations.
 """
 import datetime
 import math
@@ -38,10 +43,10 @@ import numpy as np
 import torch
 import torch.nn.functional as F
 
+from . import convnet
 from .optim import SvbrdfOptim
 
 RES_LAYERS = {256: 14}
-_BLUR = torch.tensor([1.0, 3.0, 3.0, 1.0])
 
 
 def _lrelu(x):
@@ -65,7 +70,10 @@ class _Dense(torch.nn.Module):
 class _ModConv(torch.nn.Module):
     """Modulated convolution for one sample: weights scaled by the style of
     w per input channel, demodulated per output channel (optional), optional
-    x2 upsampling (zero insertion + [1,3,3,1] blur, gain 4)."""
+    x2 upsampling (zero insertion + [1,3,3,1] blur, gain 4) before a 3x3
+    convolution; 3x3 layers take the block's noise / bias / activation
+    epilogue (ModulateConvBlock.forward,
+    src/higan_models/stylegan2_generator_network.py:694-755)."""
 
     def __init__(self, cin, cout, k, g, demodulate=True, upsample=False):
         super().__init__()
@@ -74,26 +82,20 @@ class _ModConv(torch.nn.Module):
         self.scale = 1.0 / math.sqrt(cin * k * k)
         self.demodulate, self.upsample, self.k = demodulate, upsample, k
 
-    def forward(self, x, w):
+    def modulated_weight(self, w):
         style = self.affine(w)                                       # [1, cin]
         wt = self.weight * self.scale * style.view(1, -1, 1, 1)
         if self.demodulate:
             wt = wt * torch.rsqrt((wt * wt).sum(dim=(1, 2, 3), keepdim=True) + 1e-8)
+        return wt
+
+    def forward(self, x, w, bias=None, noise=None, noise_strength=0.0, act="none"):
+        wt = self.modulated_weight(w)
+        if self.k == 1:
+            return F.conv2d(x, wt)
         if self.upsample:
-            x = _upsample2(x)
-        return F.conv2d(x, wt, padding=self.k // 2)
-
-
-def _upsample2(x):
-    """x2 upsampling: zeros inserted between pixels, then the separable
-    [1, 3, 3, 1] binomial filter with gain 4 (unit DC response)."""
-    n, c, h, w = x.shape
-    up = x.new_zeros(n, c, 2 * h, 2 * w)
-    up[:, :, ::2, ::2] = x
-    k = (_BLUR / _BLUR.sum() * 2.0).to(x)                            # per axis, gain 2
-    up = F.pad(up, (2, 1, 2, 1), mode="constant")
-    up = F.conv2d(up, k.view(1, 1, 1, 4).repeat(c, 1, 1, 1), groups=c)
-    return F.conv2d(up, k.view(1, 1, 4, 1).repeat(c, 1, 1, 1), groups=c)
+            x = convnet.upsample2(x)
+        return convnet.conv3x3(x, wt, bias, noise, noise_strength, act)
 
 
 class StyleGAN2Generator(torch.nn.Module):
@@ -123,6 +125,8 @@ class StyleGAN2Generator(torch.nn.Module):
             self.conv_res += [r, r]
             r *= 2
         self.bias = torch.nn.ParameterList([torch.nn.Parameter(torch.zeros(c.weight.shape[0])) for c in self.convs])
+        # frozen: read once here, not from the device at every layer of every call
+        self.noise_strength_values = [float(p) for p in self.noise_strength]
         self.register_buffer("w_avg", torch.zeros(512))
         self.to(device)
         for p in self.parameters():
@@ -150,40 +154,36 @@ class StyleGAN2Generator(torch.nn.Module):
         rgb = None
         wi = 0
         for i, conv in enumerate(self.convs):
-            x = conv(x, wp[:, min(wi, self.num_layers - 1)])
-            x = x + self.noise_strength[i] * noises[i]
-            x = _lrelu(x + self.bias[i].view(1, -1, 1, 1))
+            # conv + noise_strength * noise + bias, leaky ReLU x sqrt(2), in one kernel
+            x = conv(x, wp[:, min(wi, self.num_layers - 1)], bias=self.bias[i], noise=noises[i],
+                     noise_strength=self.noise_strength_values[i], act="lrelu")
             wi += 1
             if i == 0 or i % 2 == 0:
                 # toRGB after each resolution's last convolution, skip-summed
                 t = self.to_rgb[(i + 1) // 2](x, wp[:, min(wi, self.num_layers - 1)])
-                rgb = t if rgb is None else _upsample2(rgb) + t
+                rgb = t if rgb is None else convnet.upsample2(rgb) + t
         return rgb
 
 
 class VGGLoss(torch.nn.Module):
-    """src/descriptor.py:7-79 with seeded random VGG-19 weights."""
+    """src/descriptor.py:7-79 with seeded random VGG-19 weights; the 3x3
+    convolutions with their ReLU are :class:`convnet.Conv3x3` layers."""
 
     LAYERS = [64, 64, "P", 128, 128, "P", 256, 256, 256, 256, "P", 512, 512]   # conv1_1 .. conv4_2
-    HOOKS = (1, 3, 13, 22)                                                   # relu1_1, relu1_2, relu3_2, relu4_2
+    HOOKS = (0, 1, 5, 9)              # the convolutions whose ReLU outputs are relu1_1, relu1_2, relu3_2, relu4_2
 
     def __init__(self, device, weights=np.array([1, 1, 1, 1]) / 4, seed=4):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
-        mods, cin = [], 3
+        self.plan, self.convs, cin = [], [], 3
         for item in self.LAYERS:
             if item == "P":
-                mods.append(torch.nn.AvgPool2d(kernel_size=2))       # max pooling replaced, descriptor.py:18-20
+                self.plan.append("P")           # max pooling replaced by average pooling, descriptor.py:18-20
                 continue
-            conv = torch.nn.Conv2d(cin, item, 3, padding=1)
-            with torch.no_grad():
-                conv.weight.copy_(torch.randn(conv.weight.shape, generator=g) * math.sqrt(2.0 / (cin * 9)))
-                conv.bias.zero_()
-            mods += [conv, torch.nn.ReLU()]
+            wgt = torch.randn((item, cin, 3, 3), generator=g) * math.sqrt(2.0 / (cin * 9))
+            self.convs.append(convnet.Conv3x3(wgt.to(device), torch.zeros(item, device=device), act="relu"))
+            self.plan.append(len(self.convs) - 1)
             cin = item
-        self.net = torch.nn.Sequential(*mods).to(device)
-        for p in self.net.parameters():
-            p.requires_grad_(False)
         self.weights = [float(x) for x in weights]
         self.mean = torch.tensor([0.485, 0.456, 0.406], device=device).view(1, 3, 1, 1)
         self.std = torch.tensor([0.229, 0.224, 0.255], device=device).view(1, 3, 1, 1)
@@ -193,9 +193,12 @@ class VGGLoss(torch.nn.Module):
 
     def compute_feature_vector(self, x):
         feats = []
-        for i, layer in enumerate(self.net):
-            x = layer(x)
-            if i in self.HOOKS:
+        for step in self.plan:
+            if step == "P":
+                x = F.avg_pool2d(x, 2)
+                continue
+            x = self.convs[step](x)
+            if step in self.HOOKS:
                 feats.append(x.flatten() * self.weights[len(feats)])
         return torch.cat(feats)
 

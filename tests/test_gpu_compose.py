@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from oracle import microfacet_torch as MT
+from tests import convnet_ref as R
 from tests import parity
 
 pytestmark = pytest.mark.gpu
@@ -21,15 +22,27 @@ def setup(pkg, res=32, V=9):
     return gt, init, cam, light, P, pkg.synthetic.PLANE_SIZE_CM
 
 
+class _TorchStack:
+    """compose.VGGShapedFeatures' weights through F.conv2d in the input's dtype."""
+
+    def __init__(self, net):
+        self.net = net
+
+    def __call__(self, x):
+        return R.vgg_shaped_ref(self.net, x)
+
+
 def feature_grad(pkg, renderer_cls, tex0, gt, cam, light, P, size, dtype):
     from svbrdf_amd import compose
     res = tex0.shape[-1]
     cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+    net = compose.VGGShapedFeatures(seed=4, device=DEV)
     if renderer_cls is None:
         r = pkg.Microfacet(res, cam.shape[0], size, cl, DEV)
     else:
+        # the reference side: eager renderer and the stack's torch restatement
         r = renderer_cls(res, cam.shape[0], size, cl, DEV, dtype)
-    net = compose.VGGShapedFeatures(seed=4, device=DEV, dtype=dtype)
+        net = _TorchStack(net)
     with torch.no_grad():
         target = r.eval(torch.tensor(gt, device=DEV, dtype=dtype)[None])
     fl = compose.FeatureLoss(net, target)

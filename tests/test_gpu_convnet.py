@@ -1,0 +1,192 @@
+"""The matrix-core 3x3 convolution (csrc/conv.hip through svbrdf_amd.convnet)
+against torch's F.conv2d on the same weights in FP32 and FP64, and the
+networks built on it (the MaterialGAN generator's synthesis pass, the two VGG
+stacks) against their torch restatements (tests/convnet_ref.py).
+
+Forward criterion: every output within 2e-5 of the FP64 result relative to
+the convolution of |x| with |w| (the scale of a dot product's rounding
+error), plus 1e-7.  Gradients: norm-wise within 1e-5 of FP64 (single
+layers), or no worse than twice torch's own FP32 error where that is larger
+(deep networks)."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from tests import convnet_ref as R
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+SHAPES = [  # n, cin, cout, h, w, act, noise, bias
+    (1, 512, 512, 4, 4, "lrelu", True, True),
+    (1, 512, 512, 16, 16, "lrelu", True, True),
+    (1, 256, 128, 64, 64, "lrelu", True, True),
+    (9, 3, 64, 32, 32, "relu", False, True),
+    (2, 64, 64, 37, 29, "none", False, False),
+    (1, 13, 70, 5, 7, "none", False, True),
+    (3, 128, 256, 24, 40, "relu", False, True),
+]
+
+
+def _inputs(n, cin, cout, h, w, seed):
+    g = torch.Generator().manual_seed(seed)
+    x = torch.randn((n, cin, h, w), generator=g)
+    wt = torch.randn((cout, cin, 3, 3), generator=g) / np.sqrt(cin * 9)
+    b = torch.randn(cout, generator=g) * 0.1
+    nz = torch.randn((1, 1, h, w), generator=g)
+    return x.to(DEV), wt.to(DEV), b.to(DEV), nz.to(DEV)
+
+
+def _check_forward(y, x, wt, b, nz, ns, act, label):
+    with torch.no_grad():
+        y64 = R.conv3x3_ref(x.double(), wt.double(), b, nz, ns, act)
+        y32 = R.conv3x3_ref(x, wt, b, nz, ns, act)
+        scale = F.conv2d(x.double().abs(), wt.double().abs(), padding=1) + (abs(ns) * nz.double().abs() if nz is not None else 0)
+    err = (y.double() - y64).abs()
+    bound = 2e-5 * scale + 1e-7
+    ratio = float((err / bound).max())
+    assert ratio <= 1.0, f"{label}: error {float(err.max()):.3e}, worst at {ratio:.2f} of the bound"
+    return float((err / (scale + 1e-30)).max()), float(((y32.double() - y64).abs() / (scale + 1e-30)).max())
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{s[0]}x{s[1]}->{s[2]}@{s[3]}x{s[4]}" for s in SHAPES])
+def test_conv3x3_forward(pkg, shape):
+    from svbrdf_amd import convnet
+    n, cin, cout, h, w, act, use_noise, use_bias = shape
+    x, wt, b, nz = _inputs(n, cin, cout, h, w, seed=cin + cout + h)
+    b = b if use_bias else None
+    nz = nz if use_noise else None
+    ns = 0.37 if use_noise else 0.0
+    y = convnet.conv3x3(x, wt, b, nz, ns, act)
+    torch.cuda.synchronize()
+    assert y.shape == (n, cout, h, w)
+    e, e32 = _check_forward(y, x, wt, b, nz, ns, act, str(shape))
+    print(f"conv {shape}: max error / scale {e:.2e} (torch FP32: {e32:.2e})")
+
+
+def test_conv3x3_gradients(pkg):
+    """dL/dx (the same kernel on the flipped, transposed weights), dL/dw and
+    dL/dnoise through the leaky-ReLU epilogue against FP64 autograd."""
+    from svbrdf_amd import convnet
+    x, wt, b, nz = _inputs(2, 96, 160, 20, 28, seed=5)
+    gy = torch.randn((2, 160, 20, 28), generator=torch.Generator().manual_seed(6)).to(DEV)
+    xs, ws, ns_ = x.clone().requires_grad_(True), wt.clone().requires_grad_(True), nz.clone().requires_grad_(True)
+    (convnet.conv3x3(xs, ws, b, ns_, 0.3, "lrelu") * gy).sum().backward()
+    x64, w64, n64 = (t.double().requires_grad_(True) for t in (x, wt, nz))
+    (R.conv3x3_ref(x64, w64, b, n64, 0.3, "lrelu") * gy.double()).sum().backward()
+    for name, g, g64 in (("x", xs.grad, x64.grad), ("w", ws.grad, w64.grad), ("noise", ns_.grad, n64.grad)):
+        rel = float((g.double() - g64).norm() / g64.norm())
+        print(f"conv gradient d{name}: {rel:.2e}")
+        assert rel <= 1e-5, (name, rel)
+
+
+def test_pack_folds_style_and_demodulates(pkg):
+    from svbrdf_amd import convnet
+    g = torch.Generator().manual_seed(3)
+    w = torch.randn((40, 24, 3, 3), generator=g).to(DEV)
+    style = (torch.rand(24, generator=g) + 0.5).to(DEV)
+    p = convnet.pack3x3(w, style, scale=0.07, demodulate=True)
+    ref = w.double() * 0.07 * style.double().view(1, -1, 1, 1)
+    ref = ref * torch.rsqrt((ref * ref).sum(dim=(1, 2, 3), keepdim=True) + 1e-8)
+    got = p.double().reshape(3, 3, 24, 40).permute(3, 2, 0, 1)
+    assert float((got - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
+    pt = convnet.pack3x3(w, transpose=True)
+    got_t = pt.double().reshape(3, 3, 40, 24).permute(2, 3, 0, 1).flip(2, 3)
+    assert torch.equal(got_t, w.double())
+
+
+def test_upsample2_and_adjoint(pkg):
+    from svbrdf_amd import convnet
+    g = torch.Generator().manual_seed(8)
+    x = torch.randn((2, 5, 9, 13), generator=g).to(DEV)
+    gy = torch.randn((2, 5, 18, 26), generator=g).to(DEV)
+    xs = x.clone().requires_grad_(True)
+    y = convnet.upsample2(xs)
+    (y * gy).sum().backward()
+    x64 = x.double().requires_grad_(True)
+    y64 = R.upsample2_ref(x64)
+    (y64 * gy.double()).sum().backward()
+    assert float((y.double() - y64).abs().max()) <= 1e-6
+    assert float((xs.grad.double() - x64.grad).abs().max()) <= 1e-5
+    # unit DC gain away from the border
+    up = convnet.upsample2(torch.ones(1, 2, 8, 8, device=DEV))
+    assert torch.allclose(up[:, :, 1:-1, 1:-1], torch.ones_like(up[:, :, 1:-1, 1:-1]))
+
+
+def _gradient_check(g, g32, g64, label):
+    e = float((g.double() - g64).norm() / g64.norm())
+    e32 = float((g32.double() - g64).norm() / g64.norm())
+    print(f"{label}: |g - g64|/|g64| {e:.2e} (torch FP32 {e32:.2e})")
+    assert e <= max(1e-5, 2 * e32), (label, e, e32)
+
+
+def test_generator_synthesis_and_gradients(pkg):
+    """The 256^2 StyleGAN2-shaped synthesis on the HIP convolutions against its
+    torch restatement in FP32 and FP64: the 9-channel output, and the
+    gradients with respect to the W+ latent and the 14 noise maps (what
+    MaterialGANOptim optimises)."""
+    from svbrdf_amd import materialgan as MG
+    gen = MG.StyleGAN2Generator(256, 9, seed=1, device=DEV)
+    gz = torch.Generator().manual_seed(2)
+    wp = gen.truncation(gen.mapping(torch.randn(1, 512, generator=gz).to(DEV)))
+    noises = gen.make_noises(torch.Generator().manual_seed(3))
+    probe = torch.randn((1, 9, 256, 256), generator=torch.Generator().manual_seed(4)).to(DEV)
+
+    def run(fn, dtype):
+        w = wp.detach().to(dtype).requires_grad_(True)
+        ns = [n.detach().to(dtype).requires_grad_(True) for n in noises]
+        out = fn(w, ns)
+        (out * probe.to(dtype)).sum().backward()
+        return out.detach(), w.grad, [n.grad for n in ns]
+
+    out, gw, gn = run(gen.synthesis, torch.float32)
+    out32, gw32, gn32 = run(lambda w, n: R.synthesis_ref(gen, w, n), torch.float32)
+    out64, gw64, gn64 = run(lambda w, n: R.synthesis_ref(gen, w, n), torch.float64)
+    torch.cuda.synchronize()
+    assert out.shape == (1, 9, 256, 256) and torch.isfinite(out).all()
+    eo = float((out.double() - out64).abs().max() / out64.abs().max())
+    eo32 = float((out32.double() - out64).abs().max() / out64.abs().max())
+    print(f"synthesis output: max error {eo:.2e} of max |out| (torch FP32 {eo32:.2e})")
+    assert eo <= max(1e-5, 2 * eo32)
+    _gradient_check(gw, gw32, gw64, "dL/dW+")
+    _gradient_check(torch.cat([g.flatten() for g in gn]), torch.cat([g.flatten() for g in gn32]),
+                    torch.cat([g.flatten() for g in gn64]), "dL/dnoise")
+
+
+@pytest.mark.parametrize("which", ["vgg_loss", "vgg_shaped"])
+def test_vgg_stacks_and_input_gradient(pkg, which):
+    """The two VGG stacks on the HIP convolutions (9 images of 256^2, as the
+    MaterialGAN and feature-loss legs run them): features and dL/dimages
+    against the torch restatement in FP32 and FP64."""
+    from svbrdf_amd import compose, materialgan as MG
+    g = torch.Generator().manual_seed(9)
+    ims = torch.rand((9, 3, 256, 256), generator=g).to(DEV)
+    if which == "vgg_loss":
+        net = MG.VGGLoss(DEV, np.array([1, 1, 4, 8]) / 14)
+        fwd = lambda x: net.compute_feature_vector(net.normalize(x))   # noqa: E731
+        ref = lambda x: R.vgg_features_ref(net, x)                     # noqa: E731
+    else:
+        net = compose.VGGShapedFeatures(seed=4, device=DEV)
+        fwd = lambda x: torch.cat([f.flatten() for f in net(x)])       # noqa: E731
+        ref = lambda x: torch.cat([f.flatten() for f in R.vgg_shaped_ref(net, x)])  # noqa: E731
+    probe = None
+
+    def run(fn, dtype):
+        nonlocal probe
+        x = ims.detach().to(dtype).clone().requires_grad_(True)
+        f = fn(x)
+        if probe is None:
+            probe = torch.randn(f.shape, generator=torch.Generator().manual_seed(10)).to(DEV)
+        (f * probe.to(dtype)).sum().backward()
+        return f.detach(), x.grad
+
+    f, gx = run(fwd, torch.float32)
+    f32, gx32 = run(ref, torch.float32)
+    f64, gx64 = run(ref, torch.float64)
+    torch.cuda.synchronize()
+    ef = float((f.double() - f64).norm() / f64.norm())
+    ef32 = float((f32.double() - f64).norm() / f64.norm())
+    print(f"{which} features: {ef:.2e} (torch FP32 {ef32:.2e})")
+    assert ef <= max(1e-5, 2 * ef32)
+    _gradient_check(gx, gx32, gx64, f"{which} dL/dimages")
