@@ -158,6 +158,16 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
                               ? xi[((size_t)ci * H + yy) * W + xx] : 0.0f;
         }
         __syncthreads();
+        // the chunk's 72 products per output are summed in a fresh accumulator
+        // and then added to the running sum: the rounding error grows with
+        // 72 + Cin / 8 terms instead of 9 Cin
+        v16f part[TM][TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) part[i][j][r] = 0.0f;
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int boff = (t / 3) * tw2 + (t % 3);
@@ -173,9 +183,13 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int j = 0; j < TN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+                        part[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], part[i][j], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] += part[i][j];
         __syncthreads();
     }
 

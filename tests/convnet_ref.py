@@ -19,21 +19,32 @@ import torch.nn.functional as F
 _BLUR = torch.tensor([1.0, 3.0, 3.0, 1.0])
 
 
-def act_ref(v, act):
+def act_ref(v, act, mask=None):
+    """The activation; with `mask` (a boolean tensor like v) the branch of
+    each element is taken from the mask instead of the sign of v."""
+    if mask is None:
+        if act == "relu":
+            return torch.relu(v)
+        if act == "lrelu":
+            return F.leaky_relu(v, 0.2) * math.sqrt(2.0)
+        return v
     if act == "relu":
-        return torch.relu(v)
+        return v * mask.to(v.dtype)
     if act == "lrelu":
-        return F.leaky_relu(v, 0.2) * math.sqrt(2.0)
+        return torch.where(mask, v, 0.2 * v) * math.sqrt(2.0)
     return v
 
 
-def conv3x3_ref(x, w, bias=None, noise=None, noise_strength=0.0, act="none"):
+def conv3x3_ref(x, w, bias=None, noise=None, noise_strength=0.0, act="none", mask=None, record=None):
+    """With `record` (a list) the pre-activation is appended to it."""
     y = F.conv2d(x, w.to(x.dtype), padding=1)
     if noise is not None:
         y = y + noise_strength * noise.to(x.dtype)
     if bias is not None:
         y = y + bias.to(x.dtype).view(1, -1, 1, 1)
-    return act_ref(y, act)
+    if record is not None:
+        record.append(y.detach())
+    return act_ref(y, act, mask)
 
 
 def upsample2_ref(x):
@@ -58,9 +69,9 @@ def modulated_weight_ref(conv, w):
     return wt
 
 
-def synthesis_ref(gen, wp, noises):
+def synthesis_ref(gen, wp, noises, masks=None, record=None):
     """StyleGAN2Generator.synthesis in the dtype of wp (the noise maps are
-    cast to it)."""
+    cast to it); masks / record as conv3x3_ref, one per 3x3 layer."""
     dt = wp.dtype
     x = gen.const.to(dt)
     rgb = None
@@ -69,7 +80,8 @@ def synthesis_ref(gen, wp, noises):
         wt = modulated_weight_ref(conv, wp[:, min(wi, gen.num_layers - 1)])
         if conv.upsample:
             x = upsample2_ref(x)
-        x = conv3x3_ref(x, wt, gen.bias[i], noises[i].to(dt), gen.noise_strength_values[i], "lrelu")
+        x = conv3x3_ref(x, wt, gen.bias[i], noises[i].to(dt), gen.noise_strength_values[i], "lrelu",
+                        None if masks is None else masks[i], record)
         wi += 1
         if i == 0 or i % 2 == 0:
             to_rgb = gen.to_rgb[(i + 1) // 2]
@@ -78,8 +90,9 @@ def synthesis_ref(gen, wp, noises):
     return rgb
 
 
-def vgg_shaped_ref(net, x):
-    """compose.VGGShapedFeatures.forward in x's dtype."""
+def vgg_shaped_ref(net, x, masks=None, record=None):
+    """compose.VGGShapedFeatures.forward in x's dtype; masks / record as
+    conv3x3_ref, one per convolution."""
     from svbrdf_amd import compose
     feats = []
     for step in net.plan:
@@ -87,14 +100,16 @@ def vgg_shaped_ref(net, x):
             x = F.avg_pool2d(x, 2)
         else:
             c = net.convs[step]
-            x = conv3x3_ref(x, c.weight, c.bias, act="relu")
+            x = conv3x3_ref(x, c.weight, c.bias, act="relu", mask=None if masks is None else masks[step],
+                            record=record)
             if step in compose.STYLE_LAYERS:
                 feats.append(x)
     return feats
 
 
-def vgg_features_ref(vgg, x):
-    """materialgan.VGGLoss.compute_feature_vector(normalize(x)) in x's dtype."""
+def vgg_features_ref(vgg, x, masks=None, record=None):
+    """materialgan.VGGLoss.compute_feature_vector(normalize(x)) in x's dtype;
+    masks / record as conv3x3_ref, one per convolution."""
     x = (x - vgg.mean.to(x.dtype)) / vgg.std.to(x.dtype)
     feats = []
     for step in vgg.plan:
@@ -102,7 +117,7 @@ def vgg_features_ref(vgg, x):
             x = F.avg_pool2d(x, 2)
             continue
         c = vgg.convs[step]
-        x = conv3x3_ref(x, c.weight, c.bias, act="relu")
+        x = conv3x3_ref(x, c.weight, c.bias, act="relu", mask=None if masks is None else masks[step], record=record)
         if step in vgg.HOOKS:
             feats.append(x.flatten() * vgg.weights[len(feats)])
     return torch.cat(feats)

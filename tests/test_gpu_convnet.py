@@ -5,9 +5,13 @@ stacks) against their torch restatements (tests/convnet_ref.py).
 
 Forward criterion: every output within 2e-5 of the FP64 result relative to
 the convolution of |x| with |w| (the scale of a dot product's rounding
-error), plus 1e-7.  Gradients: norm-wise within 1e-5 of FP64 (single
-layers), or no worse than twice torch's own FP32 error where that is larger
-(deep networks)."""
+error), plus 1e-7.  Gradients: norm-wise within 1e-5 of FP64.  Through a
+deep stack of ReLU / leaky-ReLU layers the FP32 forward lands on the other
+side of a kink from FP64 at a handful of pre-activations within rounding of
+0 (torch's FP32 convolutions as well), and each such branch flip changes the
+gradient by a finite step; there the FP64 gradient is taken with the FP32
+forward's branches, and the flips themselves are bounded (at most 1e-6 of
+the activations, each within 1e-4 of its layer's largest pre-activation)."""
 import numpy as np
 import pytest
 import torch
@@ -107,25 +111,55 @@ def test_upsample2_and_adjoint(pkg):
     x64 = x.double().requires_grad_(True)
     y64 = R.upsample2_ref(x64)
     (y64 * gy.double()).sum().backward()
-    assert float((y.double() - y64).abs().max()) <= 1e-6
+    assert float((y.detach().double() - y64.detach()).abs().max()) <= 1e-6
     assert float((xs.grad.double() - x64.grad).abs().max()) <= 1e-5
     # unit DC gain away from the border
     up = convnet.upsample2(torch.ones(1, 2, 8, 8, device=DEV))
     assert torch.allclose(up[:, :, 1:-1, 1:-1], torch.ones_like(up[:, :, 1:-1, 1:-1]))
 
 
-def _gradient_check(g, g32, g64, label):
-    e = float((g.double() - g64).norm() / g64.norm())
+def _masks_of(layers):
+    """Forward hooks recording each layer's activation branch (output > 0,
+    the sign of the pre-activation); returns (masks, remove)."""
+    masks = []
+    handles = [m.register_forward_hook(lambda mod, inp, out: masks.append(out.detach() > 0)) for m in layers]
+    return masks, lambda: [h.remove() for h in handles]
+
+
+def _check_kinks(masks, pre64, label):
+    """The activation branches of the FP32 forward may differ from FP64's only
+    at pre-activations within rounding of 0: at most 1e-6 of them (and 16),
+    each within 1e-4 of its layer's largest |pre-activation|."""
+    flips = total = 0
+    for m, pre in zip(masks, pre64):
+        d = m != (pre > 0)
+        total += d.numel()
+        if d.any():
+            flips += int(d.sum())
+            near = float(pre[d].abs().max() / pre.abs().max())
+            assert near <= 1e-4, f"{label}: a branch flip at |pre| = {near:.2e} of the layer's largest"
+    assert flips <= max(16, 1e-6 * total), f"{label}: {flips} branch flips in {total} activations"
+    return flips, total
+
+
+def _gradient_check(g, g64m, g64, g32, label):
+    """g against the FP64 gradient taken with the same activation branches
+    (g64m); the distance to the plain FP64 gradient (g64), all of it from the
+    kinks, is reported beside torch FP32's."""
+    e = float((g.double() - g64m).norm() / g64m.norm())
+    e_all = float((g.double() - g64).norm() / g64.norm())
     e32 = float((g32.double() - g64).norm() / g64.norm())
-    print(f"{label}: |g - g64|/|g64| {e:.2e} (torch FP32 {e32:.2e})")
-    assert e <= max(1e-5, 2 * e32), (label, e, e32)
+    print(f"{label}: |g - g64(same branches)|/|g64| {e:.2e}; |g - g64| {e_all:.2e} (torch FP32 {e32:.2e})")
+    assert e <= 1e-5, (label, e)
 
 
 def test_generator_synthesis_and_gradients(pkg):
     """The 256^2 StyleGAN2-shaped synthesis on the HIP convolutions against its
     torch restatement in FP32 and FP64: the 9-channel output, and the
     gradients with respect to the W+ latent and the 14 noise maps (what
-    MaterialGANOptim optimises)."""
+    MaterialGANOptim optimises), the latter against FP64 with the same
+    leaky-ReLU branches (tests/test_gpu_convnet.py _check_kinks bounds where
+    the branches may differ)."""
     from svbrdf_amd import materialgan as MG
     gen = MG.StyleGAN2Generator(256, 9, seed=1, device=DEV)
     gz = torch.Generator().manual_seed(2)
@@ -138,38 +172,44 @@ def test_generator_synthesis_and_gradients(pkg):
         ns = [n.detach().to(dtype).requires_grad_(True) for n in noises]
         out = fn(w, ns)
         (out * probe.to(dtype)).sum().backward()
-        return out.detach(), w.grad, [n.grad for n in ns]
+        return out.detach(), w.grad, torch.cat([n.grad.flatten() for n in ns])
 
+    masks, remove = _masks_of(gen.convs)
     out, gw, gn = run(gen.synthesis, torch.float32)
+    remove()
+    pre64 = []
+    out64, gw64, gn64 = run(lambda w, n: R.synthesis_ref(gen, w, n, record=pre64), torch.float64)
+    _, gw64m, gn64m = run(lambda w, n: R.synthesis_ref(gen, w, n, masks=masks), torch.float64)
     out32, gw32, gn32 = run(lambda w, n: R.synthesis_ref(gen, w, n), torch.float32)
-    out64, gw64, gn64 = run(lambda w, n: R.synthesis_ref(gen, w, n), torch.float64)
     torch.cuda.synchronize()
     assert out.shape == (1, 9, 256, 256) and torch.isfinite(out).all()
     eo = float((out.double() - out64).abs().max() / out64.abs().max())
     eo32 = float((out32.double() - out64).abs().max() / out64.abs().max())
     print(f"synthesis output: max error {eo:.2e} of max |out| (torch FP32 {eo32:.2e})")
     assert eo <= max(1e-5, 2 * eo32)
-    _gradient_check(gw, gw32, gw64, "dL/dW+")
-    _gradient_check(torch.cat([g.flatten() for g in gn]), torch.cat([g.flatten() for g in gn32]),
-                    torch.cat([g.flatten() for g in gn64]), "dL/dnoise")
+    flips, total = _check_kinks(masks, pre64, "synthesis")
+    print(f"synthesis: {flips} leaky-ReLU branch flips in {total} activations")
+    _gradient_check(gw, gw64m, gw64, gw32, "dL/dW+")
+    _gradient_check(gn, gn64m, gn64, gn32, "dL/dnoise")
 
 
 @pytest.mark.parametrize("which", ["vgg_loss", "vgg_shaped"])
 def test_vgg_stacks_and_input_gradient(pkg, which):
     """The two VGG stacks on the HIP convolutions (9 images of 256^2, as the
-    MaterialGAN and feature-loss legs run them): features and dL/dimages
-    against the torch restatement in FP32 and FP64."""
+    MaterialGAN and feature-loss legs run them): features against the torch
+    restatement in FP32 and FP64, dL/dimages against FP64 with the same ReLU
+    branches (with the branch flips bounded as in the generator test)."""
     from svbrdf_amd import compose, materialgan as MG
     g = torch.Generator().manual_seed(9)
     ims = torch.rand((9, 3, 256, 256), generator=g).to(DEV)
     if which == "vgg_loss":
         net = MG.VGGLoss(DEV, np.array([1, 1, 4, 8]) / 14)
         fwd = lambda x: net.compute_feature_vector(net.normalize(x))   # noqa: E731
-        ref = lambda x: R.vgg_features_ref(net, x)                     # noqa: E731
+        ref = lambda x, **k: R.vgg_features_ref(net, x, **k)           # noqa: E731
     else:
         net = compose.VGGShapedFeatures(seed=4, device=DEV)
         fwd = lambda x: torch.cat([f.flatten() for f in net(x)])       # noqa: E731
-        ref = lambda x: torch.cat([f.flatten() for f in R.vgg_shaped_ref(net, x)])  # noqa: E731
+        ref = lambda x, **k: torch.cat([f.flatten() for f in R.vgg_shaped_ref(net, x, **k)])  # noqa: E731
     probe = None
 
     def run(fn, dtype):
@@ -181,12 +221,18 @@ def test_vgg_stacks_and_input_gradient(pkg, which):
         (f * probe.to(dtype)).sum().backward()
         return f.detach(), x.grad
 
+    masks, remove = _masks_of(net.convs)
     f, gx = run(fwd, torch.float32)
+    remove()
+    pre64 = []
+    f64, gx64 = run(lambda x: ref(x, record=pre64), torch.float64)
+    _, gx64m = run(lambda x: ref(x, masks=masks), torch.float64)
     f32, gx32 = run(ref, torch.float32)
-    f64, gx64 = run(ref, torch.float64)
     torch.cuda.synchronize()
     ef = float((f.double() - f64).norm() / f64.norm())
     ef32 = float((f32.double() - f64).norm() / f64.norm())
     print(f"{which} features: {ef:.2e} (torch FP32 {ef32:.2e})")
     assert ef <= max(1e-5, 2 * ef32)
-    _gradient_check(gx, gx32, gx64, f"{which} dL/dimages")
+    flips, total = _check_kinks(masks, pre64, which)
+    print(f"{which}: {flips} ReLU branch flips in {total} activations")
+    _gradient_check(gx, gx64m, gx64, gx32, f"{which} dL/dimages")
