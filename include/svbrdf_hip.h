@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 4
+#define SVBRDF_ABI_VERSION 5
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -369,10 +369,16 @@ int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *sty
  * [cout] or NULL, noise [height, width] (shared by the n images) or NULL, y
  * [n, cout, height, width].  The noise and bias terms and the activation are
  * ModulateConvBlock's (stylegan2_generator_network.py:738-755) or VGG's
- * (bias, ReLU). */
+ * (bias, ReLU).  Shapes with few output tiles (small images, many input
+ * channels) split the channel sum into slices whose partial sums need a
+ * workspace of svbrdf_conv3x3_workspace_bytes(n, cin, height, width, cout)
+ * bytes (0 for most shapes: workspace may then be NULL; a smaller one
+ * returns SVBRDF_ESPACE).  The slices are added in a fixed order: results do
+ * not depend on scheduling. */
+size_t svbrdf_conv3x3_workspace_bytes(int n, int cin, int height, int width, int cout);
 int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const float *packed, int cout,
                    const float *bias, const float *noise, float noise_strength, int activation, float *y,
-                   void *stream);
+                   void *workspace, size_t workspace_bytes, void *stream);
 
 /* x2 upsampling of `planes` planes of height x width: zero insertion and the
  * [1, 3, 3, 1] binomial filter with gain 4 (UpsamplingLayer,

@@ -65,8 +65,12 @@ def conv3x3_packed(x, packed, cout, bias=None, noise=None, noise_strength=0.0, a
         if noise.numel() != h * w:
             raise ValueError(f"noise must have {h * w} values, got {noise.numel()}")
     y = torch.empty((n, cout, h, w), dtype=torch.float32, device=x.device)
+    # K-split partial sums of small images (torch's caching allocator makes
+    # this a free-list lookup after the first call)
+    ws_bytes = int(_lib.query("svbrdf_conv3x3_workspace_bytes", n, cin, h, w, cout))
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=x.device) if ws_bytes else None
     _lib.call("svbrdf_conv3x3", _lib.ptr(x), n, cin, h, w, _lib.ptr(packed), cout, _lib.ptr(bias), _lib.ptr(noise),
-              float(noise_strength), ACT[act], _lib.ptr(y), _lib.stream_handle(x.device))
+              float(noise_strength), ACT[act], _lib.ptr(y), _lib.ptr(ws), ws_bytes, _lib.stream_handle(x.device))
     return y
 
 

@@ -16,19 +16,33 @@
 //                         stride 1, zero padding 1, as an implicit GEMM:
 //                         M = output channels, N = a 2-D tile of output
 //                         pixels, K = input channels x 9 taps, accumulated
-//                         by v_mfma_f32_32x32x2f32.
+//                         by v_mfma_f32_32x32x2f32.  Where the tiles alone
+//                         leave the device short of blocks (small images,
+//                         many channels) K is split into slices whose
+//                         partial sums go to the caller's workspace;
+//   conv_split_sum_kernel then adds the slices in order and applies the
+//                         epilogue.
+//   conv3x3_direct_kernel the same convolution for at most 8 output channels
+//                         (the input gradient of a layer that reads the RGB
+//                         image): two pixels per thread, all output channels
+//                         in registers, on the vector ALUs -- an MFMA tile of
+//                         64 output channels would be mostly padding.
 //   upsample2_blur_kernel x2 upsampling by zero insertion and the [1, 3, 3, 1]
 //                         binomial filter (gain 4): out(2p) = (x(p-1) + 3 x(p))/4,
 //                         out(2p+1) = (3 x(p) + x(p+1))/4 per axis, zero outside
 //                         the image; and its adjoint.
 //
-// The convolution stages, per chunk of 8 input channels, the chunk's
-// weights (8 x 9 x BM floats) and the input tile with a one-pixel halo in
-// LDS; every wave then runs 9 taps x 4 K-steps of (BM/64) x (BN/64) MFMA
-// tiles of 32 x 32 from LDS.  The input gradient of a convolution is the same
-// convolution of dL/dy with the flipped, transposed weights.
+// The convolution stages, per chunk of KC (4) input channels, the chunk's
+// weights (KC x 9 x BM floats) and the input tile with a one-pixel halo in
+// one of two LDS buffers (the next chunk's global loads in flight meanwhile);
+// every wave then runs 9 taps x KC/2 K-steps of (BM/64) x (BN/64) MFMA tiles
+// of 32 x 32 from LDS.  At KC = 4 a block holds 45 KB of LDS, so two blocks
+// (the register limit of the 128 x 128 tile) share a compute unit.  The
+// input gradient of a convolution is the same convolution of dL/dy with the
+// flipped, transposed weights.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "common.hpp"
@@ -42,7 +56,6 @@ namespace {
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;   // four waves
-constexpr int kKC = 8;          // input channels per staged K chunk
 constexpr int kMaxChannels = 4096;
 constexpr int kMaxSide = 8192;
 
@@ -93,8 +106,10 @@ struct ConvArgs {
     float noise_strength;
     int act;              // SVBRDF_ACT_*
     float *y;             // [N, Cout, H, W]
-    int cin, cout, h, w_;
+    float *part;          // [slices][N, Cout, H, W] partial sums when K is split, else null
+    int n, cin, cout, h, w_;
     int tw, th, tiles_x;  // pixel tile: th rows x tw columns, tw * th = BN
+    int slices, cps;      // K slices (blockIdx.z = image x slices + slice) of cps chunks each
 };
 
 // LDS floats per staged channel of the input tile: (th + 2) (tw + 2) =
@@ -108,21 +123,92 @@ __device__ __forceinline__ float activate(float v, int act) {
     return v;
 }
 
-template <int BM, int BN>
+// the epilogue of output channel co at pixel p: noise, bias, activation
+__device__ __forceinline__ float finish(const ConvArgs &a, float v, int co, size_t p) {
+    if (a.noise) v = v + a.noise_strength * a.noise[p];
+    if (a.bias) v = v + a.bias[co];
+    return activate(v, a.act);
+}
+
+// Block tile BM output channels x BN pixels (th rows x tw columns), four
+// waves of (BM/2) x (BN/2).  K runs over chunks of KC input channels; each
+// chunk stages its weights (KC x 9 x BM floats) and its input tile with a
+// one-pixel halo (KC x (th+2)(tw+2) floats) in one of two LDS buffers.  The
+// global loads of chunk c + 1 are issued into registers before the MFMAs of
+// chunk c and written to the other buffer after them, so their latency is
+// hidden by the chunk's 9 taps x KC/2 K-steps of MFMA work and one barrier
+// per chunk separates the buffers' uses.  VEC_A loads the weights as float4
+// (Cout a multiple of 4).
+template <int BM, int BN, int KC, bool VEC_A>
 __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
     constexpr int TM = BM / 64, TN = BN / 64;   // 32 x 32 MFMA tiles per wave along M and N
     constexpr int HF = halo_floats<BN>();
-    __shared__ float As[kKC][9][BM];
-    __shared__ float Bs[kKC][HF];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int AV = VEC_A ? 4 : 1;                       // floats per weight load
+    constexpr int A_LOADS = KC * 9 * BM / AV;               // weight loads per chunk
+    constexpr int A_PER = (A_LOADS + kThreads - 1) / kThreads;
+    constexpr int B_PER = (KC * HF + kThreads - 1) / kThreads;
+    __shared__ float As[2][KC * 9 * BM];
+    __shared__ float Bs[2][KC * HF];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;    // the wave's quarter of the block tile
-    const int img = blockIdx.z;
+    const int img = blockIdx.z / a.slices, slice = blockIdx.z - img * a.slices;
     const int co0 = blockIdx.y * BM;
     const int tyi = blockIdx.x / a.tiles_x, txi = blockIdx.x - tyi * a.tiles_x;
     const int y0 = tyi * a.th, x0 = txi * a.tw;
     const int tw2 = a.tw + 2, hs = (a.th + 2) * tw2;
-    const int H = a.h, W = a.w_;
-    const float *xi = a.x + (size_t)img * a.cin * H * W;
+    const int H = a.h, W = a.w_, HW = H * W;
+    const float *xi = a.x + (size_t)img * a.cin * HW;
+
+    // the input-tile loads of this thread: offset within a chunk's first
+    // channel plane (or -1 outside the image / past the tile) and channel
+    int boff[B_PER], bkc[B_PER];
+#pragma unroll
+    for (int k = 0; k < B_PER; ++k) {
+        const int q = tid + k * kThreads;
+        const int kc = q / hs, rem = q - kc * hs;
+        const int r = rem / tw2, c = rem - r * tw2;
+        const int yy = y0 - 1 + r, xx = x0 - 1 + c;
+        const bool live = q < KC * hs && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        boff[k] = live ? kc * HW + yy * W + xx : -1;
+        bkc[k] = q < KC * hs ? kc : KC;         // KC: nothing to store
+    }
+    float4 areg[A_PER];
+    float breg[B_PER];
+    auto load_chunk = [&](int ci0) {
+#pragma unroll
+        for (int k = 0; k < A_PER; ++k) {
+            const int q = tid + k * kThreads;
+            const int e = q * AV;                       // float index within the chunk: (kc, t, m)
+            const int kc = e / (9 * BM), rem = e - kc * (9 * BM);
+            const int t = rem / BM, m = rem - t * BM;
+            const int ci = ci0 + kc, co = co0 + m;
+            const bool ok = q < A_LOADS && ci < a.cin && co < a.cout;
+            const float *src = a.w + ((size_t)t * a.cin + ci) * a.cout + co;
+            if constexpr (VEC_A) {
+                areg[k] = ok ? *reinterpret_cast<const float4 *>(src) : make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                areg[k].x = ok ? *src : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B_PER; ++k)
+            breg[k] = (boff[k] >= 0 && ci0 + bkc[k] < a.cin) ? xi[(size_t)ci0 * HW + boff[k]] : 0.0f;
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < A_PER; ++k) {
+            const int q = tid + k * kThreads;
+            if (q < A_LOADS) {
+                if constexpr (VEC_A) *reinterpret_cast<float4 *>(&As[buf][q * 4]) = areg[k];
+                else As[buf][q] = areg[k].x;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B_PER; ++k) {
+            const int q = tid + k * kThreads;
+            if (bkc[k] < KC) Bs[buf][q + bkc[k] * (HF - hs)] = breg[k];   // row kc of the buffer
+        }
+    };
 
     // the halo-tile offset of the pixel each lane feeds to MFMA tile j
     int pix[TN];
@@ -141,61 +227,43 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
     const int kc_lane = lane >> 5;   // which of the two K rows of a K-step this lane carries
-    for (int ci0 = 0; ci0 < a.cin; ci0 += kKC) {
-        // stage the chunk's weights: As[kc][tap][m] = w[tap][ci0 + kc][co0 + m]
-        for (int e = threadIdx.x; e < kKC * 9 * BM; e += kThreads) {
-            const int kc = e / (9 * BM), rem = e - kc * (9 * BM);
-            const int t = rem / BM, m = rem - t * BM;
-            const int ci = ci0 + kc, co = co0 + m;
-            As[kc][t][m] = (ci < a.cin && co < a.cout) ? a.w[((size_t)t * a.cin + ci) * a.cout + co] : 0.0f;
-        }
-        // stage the input tile with its halo (zero outside the image)
-        for (int e = threadIdx.x; e < kKC * hs; e += kThreads) {
-            const int kc = e / hs, rem = e - kc * hs;
-            const int r = rem / tw2, c = rem - r * tw2;
-            const int yy = y0 - 1 + r, xx = x0 - 1 + c, ci = ci0 + kc;
-            Bs[kc][rem] = (ci < a.cin && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                              ? xi[((size_t)ci * H + yy) * W + xx] : 0.0f;
-        }
-        __syncthreads();
-        // the chunk's 72 products per output are summed in a fresh accumulator
-        // and then added to the running sum: the rounding error grows with
-        // 72 + Cin / 8 terms instead of 9 Cin
-        v16f part[TM][TN];
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) part[i][j][r] = 0.0f;
+    const int n_chunks = (a.cin + KC - 1) / KC;
+    const int c_begin = slice * a.cps;
+    const int c_end = min(n_chunks, c_begin + a.cps);
+    load_chunk(c_begin * KC);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = c_begin; c < c_end; ++c) {
+        const int buf = (c - c_begin) & 1;
+        if (c + 1 < c_end) load_chunk((c + 1) * KC);
+        const float *A = As[buf];
+        const float *B = Bs[buf];
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
-            const int boff = (t / 3) * tw2 + (t % 3);
+            const int bo = (t / 3) * tw2 + (t % 3);
 #pragma unroll
-            for (int ks = 0; ks < kKC / 2; ++ks) {
+            for (int ks = 0; ks < KC / 2; ++ks) {
                 const int kc = 2 * ks + kc_lane;
                 float av[TM], bv[TN];
 #pragma unroll
-                for (int i = 0; i < TM; ++i) av[i] = As[kc][t][wm * (BM / 2) + i * 32 + (lane & 31)];
+                for (int i = 0; i < TM; ++i) av[i] = A[(kc * 9 + t) * BM + wm * (BM / 2) + i * 32 + (lane & 31)];
 #pragma unroll
-                for (int j = 0; j < TN; ++j) bv[j] = Bs[kc][pix[j] + boff];
+                for (int j = 0; j < TN; ++j) bv[j] = B[kc * HF + pix[j] + bo];
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int j = 0; j < TN; ++j)
-                        part[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], part[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i], bv[j], acc[i][j], 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j) acc[i][j] += part[i][j];
+        if (c + 1 < c_end) store_chunk(buf ^ 1);
         __syncthreads();
     }
 
     // epilogue: element r of tile (i, j) is row m = 8 (r / 4) + 4 (lane / 32) + r % 4
-    // and column n = lane % 32 of that 32 x 32 tile
-    float *yi = a.y + (size_t)img * a.cout * H * W;
+    // and column n = lane % 32 of that 32 x 32 tile.  A K slice stores its raw
+    // partial sums instead.
+    float *yi = a.part ? a.part + ((size_t)slice * a.n + img) * a.cout * HW : a.y + (size_t)img * a.cout * HW;
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
         const int nn = wn * (BN / 2) + j * 32 + (lane & 31);
@@ -203,20 +271,95 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
         const int yy = y0 + ty, xx = x0 + tx;
         if (yy >= H || xx >= W) continue;
         const size_t p = (size_t)yy * W + xx;
-        const float nz = a.noise ? a.noise_strength * a.noise[p] : 0.0f;
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int co = co0 + wm * (BM / 2) + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 if (co >= a.cout) continue;
-                float v = acc[i][j][r];
-                if (a.noise) v = v + nz;
-                if (a.bias) v = v + a.bias[co];
-                yi[(size_t)co * H * W + p] = activate(v, a.act);
+                yi[(size_t)co * HW + p] = a.part ? acc[i][j][r] : finish(a, acc[i][j][r], co, p);
             }
         }
     }
+}
+
+// Sum of the K slices' partial sums, in slice order, and the epilogue; one
+// output element per thread.
+__global__ __launch_bounds__(kThreads) void conv_split_sum_kernel(ConvArgs a) {
+    const size_t HW = (size_t)a.h * a.w_;
+    const size_t total = (size_t)a.n * a.cout * HW;
+    const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (e >= total) return;
+    float v = a.part[e];
+    for (int s = 1; s < a.slices; ++s) v += a.part[(size_t)s * total + e];
+    const size_t p = e % HW;
+    const int co = (int)((e / HW) % a.cout);
+    a.y[e] = finish(a, v, co, p);
+}
+
+// Direct convolution for Cout = CO <= 8: a 64 x 8 pixel block, two
+// vertically adjacent pixels per thread (12 input loads per channel for 18
+// taps), the 2 x CO sums in registers.  The weights of one input channel and
+// tap are the same for every thread: scalar loads.  Out-of-image taps load a
+// clamped in-image address and select 0, so the loop has no branches.
+template <int CO>
+__global__ __launch_bounds__(kThreads) void conv3x3_direct_kernel(ConvArgs a) {
+    const int H = a.h, W = a.w_;
+    const int xx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y0 = blockIdx.y * 8 + (threadIdx.x >> 6) * 2;
+    const int img = blockIdx.z;
+    if (xx >= W || y0 >= H) return;
+    const size_t HW = (size_t)H * W;
+    int off[4][3];
+    bool in[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int y2 = y0 + r - 1, x2 = xx + c - 1;
+            in[r][c] = y2 >= 0 && y2 < H && x2 >= 0 && x2 < W;
+            off[r][c] = min(max(y2, 0), H - 1) * W + min(max(x2, 0), W - 1);
+        }
+    float acc0[CO], acc1[CO];
+#pragma unroll
+    for (int co = 0; co < CO; ++co) acc0[co] = acc1[co] = 0.0f;
+    const float *xc = a.x + (size_t)img * a.cin * HW;
+    const size_t tap_stride = (size_t)a.cin * CO;
+#pragma unroll 2
+    for (int ci = 0; ci < a.cin; ++ci, xc += HW) {
+        float v[4][3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float raw = xc[off[r][c]];
+                v[r][c] = in[r][c] ? raw : 0.0f;
+            }
+        const float *wc = a.w + (size_t)ci * CO;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+#pragma unroll
+            for (int co = 0; co < CO; ++co) {
+                const float wv = wc[t * tap_stride + co];
+                acc0[co] += v[t / 3][t % 3] * wv;
+                acc1[co] += v[t / 3 + 1][t % 3] * wv;
+            }
+        }
+    }
+    float *yi = a.y + (size_t)img * CO * HW;
+    const size_t p0 = (size_t)y0 * W + xx;
+#pragma unroll
+    for (int co = 0; co < CO; ++co) yi[(size_t)co * HW + p0] = finish(a, acc0[co], co, p0);
+    if (y0 + 1 < H) {
+#pragma unroll
+        for (int co = 0; co < CO; ++co) yi[(size_t)co * HW + p0 + W] = finish(a, acc1[co], co, p0 + W);
+    }
+}
+
+template <int CO>
+void launch_direct(const ConvArgs &a, hipStream_t s) {
+    const dim3 grid((unsigned)((a.w_ + 63) / 64), (unsigned)((a.h + 7) / 8), (unsigned)a.n);
+    hipLaunchKernelGGL(conv3x3_direct_kernel<CO>, grid, dim3(kThreads), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,15 +431,95 @@ int pick_tile_width(int w) {
     return tw;
 }
 
+#ifndef SVBRDF_CONV_KC
+#define SVBRDF_CONV_KC 4   // input channels per staged chunk
+#endif
+constexpr int kKC = SVBRDF_CONV_KC;
+constexpr int kDirectMaxCout = 8;          // conv3x3_direct_kernel up to this many output channels
+constexpr int kMinChunksPerSlice = 2;
+constexpr int kMaxSlices = 64;
+constexpr int kComputeUnits = 256;   // MI355X
+
+// How one convolution runs: the tile shape, the tiling of the image and the
+// K split.  The workspace query and the launch share it.
+struct ConvPlan {
+    bool direct;
+    int bm, bn, tw, th, tiles_x, tiles_y, co_tiles;
+    int slices, cps;
+    size_t part_floats;   // workspace floats (0 without a K split)
+};
+
+// Estimated time (us) of a tiling with `slices` K slices.  Blocks are dealt
+// to the compute units evenly, so a unit runs ceil(blocks / CUs) of them one
+// after the other (co-resident blocks share its matrix cores); a block costs
+// its chunks plus about one chunk of prologue and epilogue, a chunk of a
+// BM x BN tile 2 BM BN 9 KC flops at ~0.45 TFLOP/s per compute unit (115
+// TFLOP/s over the device, what the 128 x 128 tile sustains) over the tile's
+// relative efficiency, and 15 % more where a unit holds a single block (one
+// wave per matrix core leaves its pipeline idle between dependent issues);
+// a split adds the sum kernel's pass over the partial
+// sums (slices reads and one write of the output at ~8 TB/s).  Measured on
+// MI355X, this ranks the tilings of the VGG-19 and StyleGAN2 layers as their
+// timings do, within ~10 %.
+double estimate_us(const ConvPlan &p, int n, size_t out_bytes) {
+    const double eff = p.bm == 128 ? (p.bn == 128 ? 1.0 : 0.95) : (p.bn == 128 ? 0.97 : 0.93);
+    const long long blocks = (long long)p.tiles_x * p.tiles_y * p.co_tiles * n * p.slices;
+    const double rounds = (double)((blocks + kComputeUnits - 1) / kComputeUnits);
+    const double chunk_us = 2.0 * p.bm * p.bn * 9 * kKC / 0.45e6 / eff;
+    double t = rounds * (p.cps + 1) * chunk_us;
+    if (blocks <= kComputeUnits) t /= 0.85;   // one block per unit: one wave per matrix core
+    if (p.slices > 1) t += (double)(p.slices + 1) * out_bytes / 8.0e6;
+    return t;
+}
+
+ConvPlan plan_conv(int n, int cin, int h, int w, int cout) {
+    ConvPlan best{};
+    best.slices = 1;
+    if (cout <= kDirectMaxCout) {
+        best.direct = true;
+        return best;
+    }
+    const int n_chunks = (cin + kKC - 1) / kKC;
+    const size_t out_bytes = (size_t)n * cout * h * w * sizeof(float);
+    const int max_slices = std::max(1, std::min({kMaxSlices, n_chunks / kMinChunksPerSlice, 65535 / n}));
+    double best_t = 0.0;
+    // 128-channel tiles only where there are that many channels; fewer slices
+    // win ties (less workspace)
+    for (int bm = 128; bm >= 64; bm -= 64) {
+        if (bm == 128 && cout < 128) continue;
+        for (int bn = 128; bn >= 64; bn -= 64) {
+            ConvPlan p{};
+            p.bm = bm;
+            p.bn = bn;
+            p.co_tiles = (cout + bm - 1) / bm;
+            p.tw = pick_tile_width(w);
+            p.th = bn / p.tw;
+            p.tiles_x = (w + p.tw - 1) / p.tw;
+            p.tiles_y = (h + p.th - 1) / p.th;
+            for (int s = 1; s <= max_slices; ++s) {
+                p.cps = (n_chunks + s - 1) / s;
+                p.slices = (n_chunks + p.cps - 1) / p.cps;   // no empty slice
+                if (p.slices != s) continue;
+                const double t = estimate_us(p, n, out_bytes);
+                if (best.bm == 0 || t < 0.98 * best_t) {
+                    best = p;
+                    best_t = t;
+                }
+            }
+        }
+    }
+    best.part_floats = best.slices > 1 ? (size_t)best.slices * n * cout * h * w : 0;
+    return best;
+}
+
 template <int BM, int BN>
-int launch_conv(ConvArgs a, int n, hipStream_t s) {
-    a.tw = pick_tile_width(a.w_);
-    a.th = BN / a.tw;
-    a.tiles_x = (a.w_ + a.tw - 1) / a.tw;
-    const int tiles_y = (a.h + a.th - 1) / a.th;
-    const dim3 grid((unsigned)(a.tiles_x * tiles_y), (unsigned)((a.cout + BM - 1) / BM), (unsigned)n);
-    hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN>), grid, dim3(kThreads), 0, s, a);
-    return check_launch("svbrdf_conv3x3");
+void launch_mfma(const ConvArgs &a, const ConvPlan &p, hipStream_t s) {
+    const dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)p.co_tiles, (unsigned)(a.n * p.slices));
+    // float4 weight loads need 16-byte aligned rows of Cout floats
+    if (a.cout % 4 == 0 && ((uintptr_t)a.w & 15u) == 0)
+        hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, true>), grid, dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, false>), grid, dim3(kThreads), 0, s, a);
 }
 
 }  // namespace
@@ -314,30 +537,59 @@ int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *sty
     return check_launch(what);
 }
 
+static bool conv_shape_ok(int n, int cin, int height, int width, int cout) {
+    return n >= 1 && n <= 65535 && cin >= 1 && cout >= 1 && cin <= kMaxChannels && cout <= kMaxChannels &&
+           height >= 1 && width >= 1 && height <= kMaxSide && width <= kMaxSide;
+}
+
+size_t svbrdf_conv3x3_workspace_bytes(int n, int cin, int height, int width, int cout) {
+    if (!conv_shape_ok(n, cin, height, width, cout)) return 0;
+    return plan_conv(n, cin, height, width, cout).part_floats * sizeof(float);
+}
+
 int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const float *packed, int cout,
                    const float *bias, const float *noise, float noise_strength, int activation, float *y,
-                   void *stream) {
+                   void *workspace, size_t workspace_bytes, void *stream) {
     const char *what = "svbrdf_conv3x3";
     if (!x || !packed || !y) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
-    if (n < 1 || n > 65535 || cin < 1 || cout < 1 || cin > kMaxChannels || cout > kMaxChannels || height < 1 ||
-        width < 1 || height > kMaxSide || width > kMaxSide)
+    if (!conv_shape_ok(n, cin, height, width, cout))
         return fail(SVBRDF_ESHAPE, "%s: n=%d cin=%d cout=%d height=%d width=%d", what, n, cin, cout, height, width);
     if (activation < SVBRDF_ACT_NONE || activation > SVBRDF_ACT_LRELU)
         return fail(SVBRDF_EINVAL, "%s: unknown activation %d", what, activation);
+    const ConvPlan p = plan_conv(n, cin, height, width, cout);
+    const size_t need = p.part_floats * sizeof(float);
+    if (need && (!workspace || workspace_bytes < need))
+        return fail(SVBRDF_ESPACE, "%s: workspace of %zu bytes, this shape needs %zu "
+                    "(svbrdf_conv3x3_workspace_bytes)", what, workspace ? workspace_bytes : (size_t)0, need);
     ConvArgs a{};
     a.x = x; a.w = packed; a.bias = bias; a.noise = noise; a.noise_strength = noise_strength; a.act = activation;
-    a.y = y; a.cin = cin; a.cout = cout; a.h = height; a.w_ = width;
+    a.y = y; a.n = n; a.cin = cin; a.cout = cout; a.h = height; a.w_ = width;
+    a.part = p.slices > 1 ? (float *)workspace : nullptr;
+    a.tw = p.tw; a.th = p.th; a.tiles_x = p.tiles_x; a.slices = p.slices; a.cps = p.cps;
     hipStream_t s = (hipStream_t)stream;
-    // 128 output channels per block where there are that many; 64-pixel tiles
-    // when 128-pixel tiles would leave the device's compute units short of
-    // blocks
-    const long long pixels = (long long)height * width * n;
-    if (cout >= 128) {
-        if (pixels * ((cout + 127) / 128) >= 128LL * 512) return launch_conv<128, 128>(a, n, s);
-        return launch_conv<128, 64>(a, n, s);
+    if (p.direct) {
+        switch (cout) {
+            case 1: launch_direct<1>(a, s); break;
+            case 2: launch_direct<2>(a, s); break;
+            case 3: launch_direct<3>(a, s); break;
+            case 4: launch_direct<4>(a, s); break;
+            case 5: launch_direct<5>(a, s); break;
+            case 6: launch_direct<6>(a, s); break;
+            case 7: launch_direct<7>(a, s); break;
+            default: launch_direct<8>(a, s); break;
+        }
+        return check_launch(what);
     }
-    if (pixels * ((cout + 63) / 64) >= 128LL * 512) return launch_conv<64, 128>(a, n, s);
-    return launch_conv<64, 64>(a, n, s);
+    if (p.bm == 128 && p.bn == 128) launch_mfma<128, 128>(a, p, s);
+    else if (p.bm == 128) launch_mfma<128, 64>(a, p, s);
+    else if (p.bn == 128) launch_mfma<64, 128>(a, p, s);
+    else launch_mfma<64, 64>(a, p, s);
+    if (p.slices > 1) {
+        const size_t total = (size_t)n * cout * height * width;
+        hipLaunchKernelGGL(conv_split_sum_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, s, a);
+    }
+    return check_launch(what);
 }
 
 int svbrdf_upsample2_blur(const float *x, int planes, int height, int width, float *y, void *stream) {

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 4 == pkg._lib.ABI_VERSION
+    assert lib.svbrdf_abi_version() == 5 == pkg._lib.ABI_VERSION
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
@@ -163,3 +163,23 @@ def test_noalloc_adjoint_refuses_sizes_beyond_its_scratch(lib):
     f = _fake()
     assert lib.svbrdf_render_backward_noalloc(f, 4097, 9, 17.0, f, f, f, f, f, f, None) == -2
     assert b"4096" in lib.svbrdf_last_error()
+
+
+def test_conv_workspace_plan(lib):
+    """svbrdf_conv3x3 splits the channel sum only where the output tiles
+    alone leave the device short of blocks: 512 -> 512 channels at 4 x 4 runs
+    in slices of the input channels whose partial sums are [slices][1, 512,
+    4, 4]; a 256^2 layer of 9 images and the direct (<= 8 output channels)
+    kernel need none; a short workspace is refused before any launch."""
+    plane = 512 * 16 * 4
+    need = lib.svbrdf_conv3x3_workspace_bytes(1, 512, 4, 4, 512)
+    assert need % plane == 0 and 2 <= need // plane <= 64
+    assert lib.svbrdf_conv3x3_workspace_bytes(9, 64, 256, 256, 64) == 0
+    assert lib.svbrdf_conv3x3_workspace_bytes(9, 64, 256, 256, 3) == 0
+    assert lib.svbrdf_conv3x3_workspace_bytes(0, 64, 8, 8, 64) == 0
+    f = _fake()
+    assert lib.svbrdf_conv3x3(f, 1, 512, 4, 4, f, 512, None, None, 0.0, 0, f, None, 0, None) == -3
+    assert b"workspace" in lib.svbrdf_last_error()
+    assert lib.svbrdf_conv3x3(f, 1, 512, 4, 4, f, 512, None, None, 0.0, 0, f, f, need - 4, None) == -3
+    assert lib.svbrdf_conv3x3(f, 1, 0, 4, 4, f, 512, None, None, 0.0, 0, f, None, 0, None) == -2
+    assert lib.svbrdf_conv3x3(f, 1, 8, 4, 4, f, 8, None, None, 0.0, 7, f, None, 0, None) == -1

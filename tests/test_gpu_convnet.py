@@ -30,6 +30,10 @@ SHAPES = [  # n, cin, cout, h, w, act, noise, bias
     (2, 64, 64, 37, 29, "none", False, False),
     (1, 13, 70, 5, 7, "none", False, True),
     (3, 128, 256, 24, 40, "relu", False, True),
+    # at most 8 output channels: the direct kernel
+    (2, 64, 3, 33, 70, "none", False, False),
+    (1, 20, 7, 9, 66, "lrelu", True, True),
+    (4, 8, 8, 17, 5, "relu", False, True),
 ]
 
 
@@ -63,6 +67,8 @@ def test_conv3x3_forward(pkg, shape):
     nz = nz if use_noise else None
     ns = 0.37 if use_noise else 0.0
     y = convnet.conv3x3(x, wt, b, nz, ns, act)
+    # the K slices of small images are added in a fixed order: repeatable bits
+    assert torch.equal(y, convnet.conv3x3(x, wt, b, nz, ns, act))
     torch.cuda.synchronize()
     assert y.shape == (n, cout, h, w)
     e, e32 = _check_forward(y, x, wt, b, nz, ns, act, str(shape))
@@ -83,6 +89,22 @@ def test_conv3x3_gradients(pkg):
         rel = float((g.double() - g64).norm() / g64.norm())
         print(f"conv gradient d{name}: {rel:.2e}")
         assert rel <= 1e-5, (name, rel)
+
+
+def test_conv3x3_refuses_a_short_workspace(pkg):
+    """A shape that splits the channel sum needs the queried workspace; one
+    float short is refused before any launch."""
+    from svbrdf_amd import _lib
+    from svbrdf_amd import convnet
+    x, wt, _, _ = _inputs(1, 512, 512, 4, 4, seed=1)
+    packed = convnet.pack3x3(wt)
+    y = torch.empty((1, 512, 4, 4), device=DEV)
+    need = int(_lib.query("svbrdf_conv3x3_workspace_bytes", 1, 512, 4, 4, 512))
+    assert need > 0
+    ws = torch.empty(need // 4, device=DEV)
+    with pytest.raises(ValueError, match="workspace"):
+        _lib.call("svbrdf_conv3x3", _lib.ptr(x), 1, 512, 4, 4, _lib.ptr(packed), 512, None, None, 0.0, 0,
+                  _lib.ptr(y), _lib.ptr(ws), need - 4, _lib.stream_handle(DEV))
 
 
 def test_pack_folds_style_and_demodulates(pkg):
