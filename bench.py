@@ -644,6 +644,51 @@ def compose_leg(pkg, dev, res=256, views=9, iters=100):
             "loss_first": first, "loss_last": last}
 
 
+def conv_layers_leg(pkg, dev, reps=20):
+    """Every 3x3 layer shape of the two networks at the legs' sizes -- the
+    VGG-19-shaped stack on 9 images of 256^2 (forward and input-gradient
+    shapes) and the 256^2 generator at batch 1 -- on the HIP convolution
+    (svbrdf_conv3x3, bias + ReLU epilogue) and on F.conv2d + ReLU (MIOpen),
+    HIP events around `reps` launches after 3 warm-ups; TFLOP/s at
+    2 * 9 * Cin * Cout * pixels."""
+    from svbrdf_amd import convnet
+    import torch.nn.functional as F
+    shapes = []
+    for cin, cout, r in [(3, 64, 256), (64, 64, 256), (64, 128, 128), (128, 128, 128), (128, 256, 64),
+                         (256, 256, 64), (256, 512, 32), (512, 512, 32), (512, 512, 16)]:
+        shapes += [("vgg", 9, cin, cout, r), ("vgg dL/dx", 9, cout, cin, r)]
+    for cin, cout, r in [(512, 512, 4), (512, 512, 8), (512, 512, 16), (512, 512, 32), (512, 512, 64),
+                         (512, 256, 128), (256, 256, 128), (256, 128, 256), (128, 128, 256)]:
+        shapes.append(("generator", 1, cin, cout, r))
+    g = torch.Generator().manual_seed(11)
+    rows, tot_hip, tot_lib = [], 0.0, 0.0
+    for net, n, cin, cout, r in shapes:
+        x = torch.randn((n, cin, r, r), generator=g).to(dev)
+        w = (torch.randn((cout, cin, 3, 3), generator=g) / (cin * 9) ** 0.5).to(dev)
+        b = torch.zeros(cout, device=dev)
+        packed = convnet.pack3x3(w)
+
+        def hip():
+            convnet.conv3x3_packed(x, packed, cout, b, None, 0.0, "relu")
+
+        def lib():
+            torch.relu(F.conv2d(x, w, b, padding=1))
+        for fn in (hip, lib):
+            for _ in range(3):
+                fn()
+        t_hip = time_events(hip, reps) * 1e3
+        t_lib = time_events(lib, reps) * 1e3
+        flops = 2.0 * 9 * cin * cout * n * r * r
+        tot_hip += t_hip
+        tot_lib += t_lib
+        rows.append({"net": net, "n": n, "cin": cin, "cout": cout, "side": r, "hip_us": round(t_hip, 1),
+                     "hip_tflops": round(flops / t_hip / 1e6, 1), "miopen_us": round(t_lib, 1),
+                     "miopen_tflops": round(flops / t_lib / 1e6, 1)})
+    return {"workload": "3x3 layers of the VGG-19-shaped stack (9 x 256^2, forward and dL/dx shapes) and the 256^2 "
+                        "StyleGAN2-shaped generator (batch 1), fp32; HIP vs F.conv2d (MIOpen)",
+            "total_hip_us": round(tot_hip, 1), "total_miopen_us": round(tot_lib, 1), "layers": rows}
+
+
 def host_cpu():
     """(threads to use, description) for the CPU legs: every core of this
     process's affinity set, capped at OMP_NUM_THREADS when the environment
@@ -740,6 +785,8 @@ def main():
     ap.add_argument("--no-compose", action="store_true", help="skip the feature-loss composition leg (256^2 x 9)")
     ap.add_argument("--no-codec", action="store_true", help="skip the map / target codec leg")
     ap.add_argument("--no-materialgan", action="store_true", help="skip the MaterialGAN caller leg")
+    ap.add_argument("--no-conv-layers", action="store_true",
+                    help="skip the per-layer HIP vs MIOpen timing of the two networks' 3x3 convolutions")
     ap.add_argument("--no-light", action="store_true", help="skip the light-power optimisation leg")
     ap.add_argument("--no-cache-cold", action="store_true",
                     help="skip the leg that keeps the step's working set out of the Infinity Cache")
@@ -889,6 +936,8 @@ def main():
         out["codec"] = codec_leg(pkg, dev, res, views)
     if not args.no_materialgan:
         out["materialgan"] = materialgan_leg(pkg, dev)
+    if not args.no_conv_layers:
+        out["conv_layers"] = conv_layers_leg(pkg, dev)
     if not args.no_driver:
         out["driver"] = driver_leg(pkg, dev, res)
     if not args.no_torch:
