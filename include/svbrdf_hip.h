@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 5
+#define SVBRDF_ABI_VERSION 6
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -379,6 +379,19 @@ size_t svbrdf_conv3x3_workspace_bytes(int n, int cin, int height, int width, int
 int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const float *packed, int cout,
                    const float *bias, const float *noise, float noise_strength, int activation, float *y,
                    void *workspace, size_t workspace_bytes, void *stream);
+
+/* dL/dx of y = act(conv3x3(x, w) + ...): the convolution of dL/dy [n, cout,
+ * height, width] gated by the activation's derivative at y (the layer's
+ * forward output: 1 / 0 for ReLU, sqrt(2) / 0.2 sqrt(2) for leaky ReLU by the
+ * sign of y; act = SVBRDF_ACT_NONE ignores y) with the flipped, transposed
+ * weights (svbrdf_conv3x3_pack with transpose != 0), written to grad_x [n,
+ * cin, height, width].  torch autograd's backward of F.conv2d + the
+ * activation (src/descriptor.py:7-79 for VGG's ReLU layers) in one pass:
+ * the gate is applied while the kernel stages its input.  Workspace:
+ * svbrdf_conv3x3_workspace_bytes(n, cout, height, width, cin). */
+int svbrdf_conv3x3_input_grad(const float *grad_y, const float *y, int activation, int n, int cout, int height,
+                              int width, const float *packed_t, int cin, float *grad_x, void *workspace,
+                              size_t workspace_bytes, void *stream);
 
 /* x2 upsampling of `planes` planes of height x width: zero insertion and the
  * [1, 3, 3, 1] binomial filter with gain 4 (UpsamplingLayer,

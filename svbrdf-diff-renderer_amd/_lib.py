@@ -15,7 +15,7 @@ _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("SVBRDF_AMD_LIB") or _PKG / "libsvbrdf_hip.so")
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OK = 0
 EINVAL = -1
@@ -86,6 +86,8 @@ SIGNATURES = {
     "svbrdf_conv3x3": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                                       ctypes.c_int, _vp, _vp, ctypes.c_float, ctypes.c_int, _vp, _vp,
                                       ctypes.c_size_t, _vp]),
+    "svbrdf_conv3x3_input_grad": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_size_t, _vp]),
     "svbrdf_upsample2_blur": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "svbrdf_upsample2_blur_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
 }

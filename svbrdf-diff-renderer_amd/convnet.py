@@ -8,8 +8,9 @@ kernels of libsvbrdf_hip.so (csrc/conv.hip).
   src/higan_models/stylegan2_generator_network.py:738-755).  Differentiable
   with respect to the input, the weights and the noise map: the forward and
   dL/dx are the HIP kernel (dL/dx is the convolution of dL/dy with the
-  flipped, transposed weights), dL/dweights is torch's weight-gradient
-  convolution.
+  flipped, transposed weights; where only dL/dx is wanted the kernel applies
+  the activation's derivative to dL/dy itself, :func:`conv3x3_input_grad`),
+  dL/dweights is torch's weight-gradient convolution.
 * :func:`upsample2` -- x2 upsampling by zero insertion and the [1, 3, 3, 1]
   filter (UpsamplingLayer, stylegan2_generator_network.py:419-479); forward
   and adjoint are HIP kernels.
@@ -74,6 +75,27 @@ def conv3x3_packed(x, packed, cout, bias=None, noise=None, noise_strength=0.0, a
     return y
 
 
+def conv3x3_input_grad(grad_y, y, packed_t, cin, act):
+    """dL/dx of y = act(conv3x3(x, w) + ...) from dL/dy and y, with the
+    flipped, transposed weights of pack3x3(w, transpose=True): the kernel
+    gates dL/dy by the activation's derivative while it stages it."""
+    g = grad_y.contiguous()
+    _check(g, "grad_y")
+    y = y.contiguous()
+    _check(y, "y")
+    n, cout, h, w = (int(s) for s in g.shape)
+    if tuple(y.shape) != tuple(g.shape):
+        raise ValueError(f"y {tuple(y.shape)} and grad_y {tuple(g.shape)} differ")
+    if tuple(packed_t.shape) != (9, cout, cin):
+        raise ValueError(f"packed weights {tuple(packed_t.shape)} do not match Cout={cout}, Cin={cin}")
+    gx = torch.empty((n, cin, h, w), dtype=torch.float32, device=g.device)
+    ws_bytes = int(_lib.query("svbrdf_conv3x3_workspace_bytes", n, cout, h, w, cin))
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=g.device) if ws_bytes else None
+    _lib.call("svbrdf_conv3x3_input_grad", _lib.ptr(g), _lib.ptr(y), ACT[act], n, cout, h, w, _lib.ptr(packed_t), cin,
+              _lib.ptr(gx), _lib.ptr(ws), ws_bytes, _lib.stream_handle(g.device))
+    return gx
+
+
 def _act_grad(grad_y, y, act):
     """dL/d(pre-activation) from dL/dy and y: the activations keep the sign."""
     if act == "relu":
@@ -97,8 +119,14 @@ class Conv3x3Function(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_y):
         x, weight, noise, y = ctx.saved_tensors
-        gz = _act_grad(grad_y.contiguous(), y, ctx.act).contiguous()
         gx = gw = gnoise = None
+        need_gz = ctx.needs_input_grad[1] or (ctx.needs_input_grad[3] and noise is not None)
+        if not need_gz:
+            # only dL/dx: the gate is applied inside the convolution
+            if ctx.needs_input_grad[0]:
+                gx = conv3x3_input_grad(grad_y, y, pack3x3(weight, transpose=True), int(weight.shape[1]), ctx.act)
+            return gx, None, None, None, None, None
+        gz = _act_grad(grad_y.contiguous(), y, ctx.act).contiguous()
         if ctx.needs_input_grad[0]:
             gx = conv3x3_packed(gz, pack3x3(weight, transpose=True), int(weight.shape[1]))
         if ctx.needs_input_grad[1]:
@@ -150,8 +178,7 @@ class _FrozenConvFunction(torch.autograd.Function):
     def backward(ctx, grad_y):
         (y,) = ctx.saved_tensors
         layer = ctx.layer
-        gz = _act_grad(grad_y.contiguous(), y, layer.act).contiguous()
-        return conv3x3_packed(gz, layer.packed_t, layer.cin), None
+        return conv3x3_input_grad(grad_y, y, layer.packed_t, layer.cin, layer.act), None
 
 
 class Conv3x3(torch.nn.Module):

@@ -105,6 +105,11 @@ struct ConvArgs {
     const float *noise;   // [H, W] or null
     float noise_strength;
     int act;              // SVBRDF_ACT_*
+    // input gate: x is read as x * (gate > 0 ? gate_pos : gate_neg) where gate
+    // is not null ([N, Cin, H, W], the forward output of the layer whose input
+    // gradient this convolution computes: the activation's derivative)
+    const float *gate;
+    float gate_pos, gate_neg;
     float *y;             // [N, Cout, H, W]
     float *part;          // [slices][N, Cout, H, W] partial sums when K is split, else null
     int n, cin, cout, h, w_;
@@ -159,8 +164,21 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
     const int H = a.h, W = a.w_, HW = H * W;
     const float *xi = a.x + (size_t)img * a.cin * HW;
 
-    // the input-tile loads of this thread: offset within a chunk's first
-    // channel plane (or -1 outside the image / past the tile) and channel
+    // The staging loads of this thread, fixed for every chunk: offsets from
+    // the chunk's first weight row / input plane (-1: nothing there) and the
+    // chunk channel they belong to.  Loads without data read a valid address
+    // (the chunk's first element) and select 0, so the staging has no branches.
+    int aoff[A_PER], akc[A_PER];
+#pragma unroll
+    for (int k = 0; k < A_PER; ++k) {
+        const int q = tid + k * kThreads;
+        const int e = q * AV;                           // float index within the chunk: (kc, t, m)
+        const int kc = e / (9 * BM), rem = e - kc * (9 * BM);
+        const int t = rem / BM, m = rem - t * BM;
+        const bool live = q < A_LOADS && co0 + m < a.cout;
+        aoff[k] = live ? (t * a.cin + kc) * a.cout + co0 + m : -1;
+        akc[k] = kc;
+    }
     int boff[B_PER], bkc[B_PER];
 #pragma unroll
     for (int k = 0; k < B_PER; ++k) {
@@ -172,41 +190,49 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
         boff[k] = live ? kc * HW + yy * W + xx : -1;
         bkc[k] = q < KC * hs ? kc : KC;         // KC: nothing to store
     }
+    // load_chunk only issues the loads (into registers); store_chunk, after
+    // the chunk's MFMAs, applies the zero selects and the gate and writes LDS,
+    // so no wait on these loads falls before the MFMAs
     float4 areg[A_PER];
-    float breg[B_PER];
+    float breg[B_PER], greg[B_PER];
+    bool aok[A_PER], bok[B_PER];
     auto load_chunk = [&](int ci0) {
+        const int lim = a.cin - ci0;            // channels left: the last chunk may be partial
+        const float *wc = a.w + (size_t)ci0 * a.cout;
+        const float *xc = xi + (size_t)ci0 * HW;
+        const float *gc = a.gate ? a.gate + (size_t)img * a.cin * HW + (size_t)ci0 * HW : nullptr;
 #pragma unroll
         for (int k = 0; k < A_PER; ++k) {
-            const int q = tid + k * kThreads;
-            const int e = q * AV;                       // float index within the chunk: (kc, t, m)
-            const int kc = e / (9 * BM), rem = e - kc * (9 * BM);
-            const int t = rem / BM, m = rem - t * BM;
-            const int ci = ci0 + kc, co = co0 + m;
-            const bool ok = q < A_LOADS && ci < a.cin && co < a.cout;
-            const float *src = a.w + ((size_t)t * a.cin + ci) * a.cout + co;
-            if constexpr (VEC_A) {
-                areg[k] = ok ? *reinterpret_cast<const float4 *>(src) : make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                areg[k].x = ok ? *src : 0.0f;
-            }
+            aok[k] = aoff[k] >= 0 && akc[k] < lim;
+            const float *src = wc + (aok[k] ? aoff[k] : 0);
+            if constexpr (VEC_A) areg[k] = *reinterpret_cast<const float4 *>(src);
+            else areg[k].x = *src;
         }
 #pragma unroll
-        for (int k = 0; k < B_PER; ++k)
-            breg[k] = (boff[k] >= 0 && ci0 + bkc[k] < a.cin) ? xi[(size_t)ci0 * HW + boff[k]] : 0.0f;
+        for (int k = 0; k < B_PER; ++k) {
+            bok[k] = boff[k] >= 0 && bkc[k] < lim;
+            const int o = bok[k] ? boff[k] : 0;
+            breg[k] = xc[o];
+            if (gc) greg[k] = gc[o];
+        }
     };
     auto store_chunk = [&](int buf) {
 #pragma unroll
         for (int k = 0; k < A_PER; ++k) {
             const int q = tid + k * kThreads;
             if (q < A_LOADS) {
-                if constexpr (VEC_A) *reinterpret_cast<float4 *>(&As[buf][q * 4]) = areg[k];
-                else As[buf][q] = areg[k].x;
+                if constexpr (VEC_A)
+                    *reinterpret_cast<float4 *>(&As[buf][q * 4]) = aok[k] ? areg[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+                else
+                    As[buf][q] = aok[k] ? areg[k].x : 0.0f;
             }
         }
 #pragma unroll
         for (int k = 0; k < B_PER; ++k) {
             const int q = tid + k * kThreads;
-            if (bkc[k] < KC) Bs[buf][q + bkc[k] * (HF - hs)] = breg[k];   // row kc of the buffer
+            float v = breg[k];
+            if (a.gate) v = v * (greg[k] > 0.0f ? a.gate_pos : a.gate_neg);
+            if (bkc[k] < KC) Bs[buf][q + bkc[k] * (HF - hs)] = bok[k] ? v : 0.0f;   // row kc of the buffer
         }
     };
 
@@ -236,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
     for (int c = c_begin; c < c_end; ++c) {
         const int buf = (c - c_begin) & 1;
         if (c + 1 < c_end) load_chunk((c + 1) * KC);
-        const float *A = As[buf];
+        const float *A = As[buf] + wm * (BM / 2) + (lane & 31);
         const float *B = Bs[buf];
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
@@ -246,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
                 const int kc = 2 * ks + kc_lane;
                 float av[TM], bv[TN];
 #pragma unroll
-                for (int i = 0; i < TM; ++i) av[i] = A[(kc * 9 + t) * BM + wm * (BM / 2) + i * 32 + (lane & 31)];
+                for (int i = 0; i < TM; ++i) av[i] = A[(kc * 9 + t) * BM + i * 32];
 #pragma unroll
                 for (int j = 0; j < TN; ++j) bv[j] = B[kc * HF + pix[j] + bo];
 #pragma unroll
@@ -324,15 +350,17 @@ __global__ __launch_bounds__(kThreads) void conv3x3_direct_kernel(ConvArgs a) {
 #pragma unroll
     for (int co = 0; co < CO; ++co) acc0[co] = acc1[co] = 0.0f;
     const float *xc = a.x + (size_t)img * a.cin * HW;
+    const float *gc = a.gate ? a.gate + (size_t)img * a.cin * HW : nullptr;
     const size_t tap_stride = (size_t)a.cin * CO;
 #pragma unroll 2
-    for (int ci = 0; ci < a.cin; ++ci, xc += HW) {
+    for (int ci = 0; ci < a.cin; ++ci, xc += HW, gc = gc ? gc + HW : nullptr) {
         float v[4][3];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float raw = xc[off[r][c]];
+                float raw = xc[off[r][c]];
+                if (gc) raw = raw * (gc[off[r][c]] > 0.0f ? a.gate_pos : a.gate_neg);
                 v[r][c] = in[r][c] ? raw : 0.0f;
             }
         const float *wc = a.w + (size_t)ci * CO;
@@ -547,28 +575,21 @@ size_t svbrdf_conv3x3_workspace_bytes(int n, int cin, int height, int width, int
     return plan_conv(n, cin, height, width, cout).part_floats * sizeof(float);
 }
 
-int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const float *packed, int cout,
-                   const float *bias, const float *noise, float noise_strength, int activation, float *y,
-                   void *workspace, size_t workspace_bytes, void *stream) {
-    const char *what = "svbrdf_conv3x3";
-    if (!x || !packed || !y) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
-    if (!conv_shape_ok(n, cin, height, width, cout))
-        return fail(SVBRDF_ESHAPE, "%s: n=%d cin=%d cout=%d height=%d width=%d", what, n, cin, cout, height, width);
-    if (activation < SVBRDF_ACT_NONE || activation > SVBRDF_ACT_LRELU)
-        return fail(SVBRDF_EINVAL, "%s: unknown activation %d", what, activation);
-    const ConvPlan p = plan_conv(n, cin, height, width, cout);
+static int run_conv(const char *what, ConvArgs a, void *workspace, size_t workspace_bytes, hipStream_t s) {
+    if (!conv_shape_ok(a.n, a.cin, a.h, a.w_, a.cout))
+        return fail(SVBRDF_ESHAPE, "%s: n=%d cin=%d cout=%d height=%d width=%d", what, a.n, a.cin, a.cout, a.h,
+                    a.w_);
+    if (a.act < SVBRDF_ACT_NONE || a.act > SVBRDF_ACT_LRELU)
+        return fail(SVBRDF_EINVAL, "%s: unknown activation %d", what, a.act);
+    const ConvPlan p = plan_conv(a.n, a.cin, a.h, a.w_, a.cout);
     const size_t need = p.part_floats * sizeof(float);
     if (need && (!workspace || workspace_bytes < need))
         return fail(SVBRDF_ESPACE, "%s: workspace of %zu bytes, this shape needs %zu "
                     "(svbrdf_conv3x3_workspace_bytes)", what, workspace ? workspace_bytes : (size_t)0, need);
-    ConvArgs a{};
-    a.x = x; a.w = packed; a.bias = bias; a.noise = noise; a.noise_strength = noise_strength; a.act = activation;
-    a.y = y; a.n = n; a.cin = cin; a.cout = cout; a.h = height; a.w_ = width;
     a.part = p.slices > 1 ? (float *)workspace : nullptr;
     a.tw = p.tw; a.th = p.th; a.tiles_x = p.tiles_x; a.slices = p.slices; a.cps = p.cps;
-    hipStream_t s = (hipStream_t)stream;
     if (p.direct) {
-        switch (cout) {
+        switch (a.cout) {
             case 1: launch_direct<1>(a, s); break;
             case 2: launch_direct<2>(a, s); break;
             case 3: launch_direct<3>(a, s); break;
@@ -585,11 +606,42 @@ int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const 
     else if (p.bn == 128) launch_mfma<64, 128>(a, p, s);
     else launch_mfma<64, 64>(a, p, s);
     if (p.slices > 1) {
-        const size_t total = (size_t)n * cout * height * width;
+        const size_t total = (size_t)a.n * a.cout * a.h * a.w_;
         hipLaunchKernelGGL(conv_split_sum_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)),
                            dim3(kThreads), 0, s, a);
     }
     return check_launch(what);
+}
+
+int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const float *packed, int cout,
+                   const float *bias, const float *noise, float noise_strength, int activation, float *y,
+                   void *workspace, size_t workspace_bytes, void *stream) {
+    const char *what = "svbrdf_conv3x3";
+    if (!x || !packed || !y) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    ConvArgs a{};
+    a.x = x; a.w = packed; a.bias = bias; a.noise = noise; a.noise_strength = noise_strength; a.act = activation;
+    a.y = y; a.n = n; a.cin = cin; a.cout = cout; a.h = height; a.w_ = width;
+    return run_conv(what, a, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int svbrdf_conv3x3_input_grad(const float *grad_y, const float *y, int activation, int n, int cout, int height,
+                              int width, const float *packed_t, int cin, float *grad_x, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+    const char *what = "svbrdf_conv3x3_input_grad";
+    if (!grad_y || !y || !packed_t || !grad_x) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (activation < SVBRDF_ACT_NONE || activation > SVBRDF_ACT_LRELU)
+        return fail(SVBRDF_EINVAL, "%s: unknown activation %d", what, activation);
+    ConvArgs a{};
+    // the convolution of dL/dy (cout channels) with the flipped, transposed
+    // weights, its input gated by the activation's derivative at y
+    a.x = grad_y; a.w = packed_t; a.act = SVBRDF_ACT_NONE; a.y = grad_x;
+    a.n = n; a.cin = cout; a.cout = cin; a.h = height; a.w_ = width;
+    if (activation != SVBRDF_ACT_NONE) {
+        a.gate = y;
+        a.gate_pos = activation == SVBRDF_ACT_LRELU ? 1.41421356237309515f : 1.0f;
+        a.gate_neg = activation == SVBRDF_ACT_LRELU ? 0.2f * 1.41421356237309515f : 0.0f;
+    }
+    return run_conv(what, a, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 int svbrdf_upsample2_blur(const float *x, int planes, int height, int width, float *y, void *stream) {

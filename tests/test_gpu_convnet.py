@@ -91,6 +91,36 @@ def test_conv3x3_gradients(pkg):
         assert rel <= 1e-5, (name, rel)
 
 
+GATED = [(2, 64, 96, 20, 28), (1, 512, 512, 8, 8), (3, 3, 48, 33, 40)]   # n, cin, cout, h, w of the layer
+
+
+@pytest.mark.parametrize("act", ["relu", "lrelu", "none"])
+@pytest.mark.parametrize("shape", GATED, ids=["mfma", "k-split", "direct"])
+def test_conv3x3_input_grad_gated(pkg, act, shape):
+    """svbrdf_conv3x3_input_grad (dL/dy gated by the activation's derivative
+    while the kernel stages it) against FP64 autograd through the same
+    activation branches as the FP32 forward, and against the two-pass path
+    (torch gate, then the plain input-gradient convolution): the MFMA
+    kernel, its K split and the direct kernel (a 3-channel input)."""
+    from svbrdf_amd import convnet
+    n, cin, cout, h, w = shape
+    x, wt, b, _ = _inputs(n, cin, cout, h, w, seed=cin + 3 * cout)
+    y = convnet.conv3x3(x, wt, b, None, 0.0, act)
+    gy = torch.randn(y.shape, generator=torch.Generator().manual_seed(12)).to(DEV)
+    packed_t = convnet.pack3x3(wt, transpose=True)
+    gx = convnet.conv3x3_input_grad(gy, y, packed_t, cin, act)
+    two_pass = convnet.conv3x3_packed(convnet._act_grad(gy, y, act).contiguous(), packed_t, cin)
+    x64 = x.double().requires_grad_(True)
+    out = R.conv3x3_ref(x64, wt.double(), b, None, 0.0, act, mask=(y > 0) if act != "none" else None)
+    (out * gy.double()).sum().backward()
+    torch.cuda.synchronize()
+    rel = float((gx.double() - x64.grad).norm() / x64.grad.norm())
+    rel2 = float((gx - two_pass).norm() / two_pass.norm())
+    print(f"gated dL/dx {shape} {act}: {rel:.2e} vs FP64, {rel2:.2e} vs the two-pass path")
+    assert rel <= 1e-5, rel
+    assert rel2 <= 1e-6, rel2
+
+
 def test_conv3x3_refuses_a_short_workspace(pkg):
     """A shape that splits the channel sum needs the queried workspace; one
     float short is refused before any launch."""
