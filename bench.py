@@ -545,6 +545,10 @@ def codec_leg(pkg, dev, res=1024, views=9, iters=20):
         f"stage transition in memory {res // 2}->{res} (svbrdf_upsample2x_bilinear)": (
             lambda: L.call("svbrdf_upsample2x_bilinear", L.ptr(half), res // 2, 0, L.ptr(tex_out), 0, 9, st),
             lambda: pkg.upsample_textures(half), 9 * 4 * (npx // 4 + npx)),
+        # the ceiling of a kernel that writes these bytes: torch's fill of the
+        # transition's output (write-only, the same 37.7 MB at 1024^2)
+        f"write-only reference: fill of the {res}^2 x 9 transition output (torch fill_)": (
+            lambda: tex_out.fill_(0.5), lambda: tex_out.fill_(0.5), 9 * 4 * npx),
     }
     out = {}
     for name, (kernel, api, nbytes) in cases.items():

@@ -15,7 +15,7 @@
 //   textures_to_u8_kernel   [9, R, R] f32 -> 4 maps u8             36 B read, 10 B written / px
 //   images_to_u8_kernel     [n, C, h, w] f32 -> [n, h, w, C] u8    4 C B read, C B written / px
 //   upsample2x_kernel       [P, R, R] f32 -> [P, 2R, 2R] f32 bilinear (stage chaining in memory);
-//   upsample2x_x4_kernel    the same, 2 x 4 outputs per thread and plane, several planes per thread
+//   upsample2x_rows_kernel  the same, a strip of 4 source rows x 2 columns (8 x 4 outputs) per thread
 //                           (even R, aligned rows)
 //
 // Float arithmetic follows the reference's float32 numpy / torch op order one
@@ -515,71 +515,69 @@ __global__ __launch_bounds__(kBlock) void upsample2x_kernel(const float *__restr
     }
 }
 
-// The same upsample, eight outputs per thread for an even res with 16-byte
-// aligned destination rows: source row i and source columns 2q, 2q + 1 give
-// output rows 2i, 2i + 1 and columns 4q .. 4q + 3.  With align_corners=False
-// and scale 2 the weights are 1/4 and 3/4 everywhere except output row /
-// column 0 (weights 1 and 0) and the neighbour index is clamped at the last
-// row / column, so a thread needs source rows i - 1, i, i + 1 and columns
-// 2q - 1 .. 2q + 2 (clamped): per plane three float2 and six scalar loads
-// and two float4 stores, instead of 32 loads and eight scalar stores.  A
-// thread takes kUpPlanes planes (blockIdx.y selects the group) and issues all
-// their loads before the first store, so that each wave has that many
-// loads in flight.  Every output is the same expression, in the same order,
-// as upsample2x_kernel evaluates it (h0 (w0 a + w1 b) + h1 (w0 c + w1 d), one
-// rounding per operation), so the two kernels agree bit for bit.
-#ifndef SVBRDF_UP_PLANES
-#define SVBRDF_UP_PLANES 3
-#endif
-constexpr int kUpPlanes = SVBRDF_UP_PLANES;
+// The same upsample for an even res with 16-byte aligned destination rows:
+// one plane per thread (blockIdx.y), source columns 2q, 2q + 1 (output
+// columns 4q .. 4q + 3) and a strip of kUpRows source rows (output rows 2i,
+// 2i + 1 for each).  With align_corners=False and scale 2 the weights are 1/4
+// and 3/4 everywhere except output row / column 0 (weights 1 and 0) and the
+// neighbour index is clamped at the last row / column, so the strip needs
+// source rows i0 - 1 .. i0 + kUpRows and columns 2q - 1 .. 2q + 2 (clamped):
+// they are loaded once (a float2 and two scalars per row, all before the first
+// store) and each row's horizontal interpolation serves the two output rows
+// either side of it, so a thread issues (kUpRows + 2) x 3 loads for
+// 2 kUpRows float4 stores.  Every output is the same expression, in the same
+// order, as upsample2x_kernel evaluates it (h0 (w0 a + w1 b) + h1 (w0 c + w1 d),
+// one rounding per operation), so the two kernels agree bit for bit.
+// kUpRows = 4 measured best at 512^2 -> 1024^2 x 9 planes (9.6 us against
+// 10.1 for 2 rows, 10.3 for 8, and 12.6 for one row pair per thread).
+constexpr int kUpRows = 4;
 
-__global__ __launch_bounds__(kBlock) void upsample2x_x4_kernel(const float *__restrict__ src, unsigned res,
-                                                               size_t src_stride, float *__restrict__ dst,
-                                                               size_t dst_stride, int n_planes) {
+__global__ __launch_bounds__(kBlock) void upsample2x_rows_kernel(const float *__restrict__ src, unsigned res,
+                                                                 size_t src_stride, float *__restrict__ dst,
+                                                                 size_t dst_stride) {
     const unsigned half = res / 2u;
+    const unsigned strips = (res + kUpRows - 1u) / kUpRows;
     const unsigned t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= res * half) return;
-    const unsigned i = t / half, q = t - i * half;
-    const unsigned last = res - 1u;
+    if (t >= strips * half) return;
+    const unsigned strip = t / half, q = t - strip * half;
+    const unsigned i0 = strip * kUpRows, last = res - 1u;
     const unsigned c0 = q == 0u ? 0u : 2u * q - 1u, c1 = 2u * q, c3 = 2u * q + 2u > last ? last : 2u * q + 2u;
-    const bool left = q == 0u, top = i == 0u;
-    const unsigned rows[3] = {top ? 0u : i - 1u, i, i + 1u > last ? last : i + 1u};
-    const int p0 = (int)blockIdx.y * kUpPlanes;
-    // every load of the thread's planes first: win[plane][row][column 2q-1 .. 2q+2]
-    float win[kUpPlanes][3][4];
+    const bool left = q == 0u;
+    const float *s = src + (size_t)blockIdx.y * src_stride;
+    // source rows i0 - 1 .. i0 + kUpRows, clamped to the image
+    float win[kUpRows + 2][4];
 #pragma unroll
-    for (int pl = 0; pl < kUpPlanes; ++pl) {
-        const int c = p0 + pl < n_planes ? p0 + pl : n_planes - 1;   // a tail group re-reads the last plane
-        const float *s = src + (size_t)c * src_stride;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const float *row = s + (size_t)rows[r] * res;
-            const float2 mid = *reinterpret_cast<const float2 *>(row + c1);
-            win[pl][r][0] = row[c0];
-            win[pl][r][1] = mid.x;
-            win[pl][r][2] = mid.y;
-            win[pl][r][3] = row[c3];
-        }
+    for (int r = 0; r < kUpRows + 2; ++r) {
+        const int ri = (int)i0 - 1 + r;
+        const unsigned row_i = ri < 0 ? 0u : ((unsigned)ri > last ? last : (unsigned)ri);
+        const float *row = s + (size_t)row_i * res;
+        const float2 mid = *reinterpret_cast<const float2 *>(row + c1);
+        win[r][0] = row[c0];
+        win[r][1] = mid.x;
+        win[r][2] = mid.y;
+        win[r][3] = row[c3];
     }
+    float h[kUpRows + 2][4];
 #pragma unroll
-    for (int pl = 0; pl < kUpPlanes; ++pl) {
-        if (p0 + pl >= n_planes) break;
-        float h[3][4];
+    for (int r = 0; r < kUpRows + 2; ++r) {
+        const float w0 = win[r][0], w1 = win[r][1], w2 = win[r][2], w3 = win[r][3];
+        h[r][0] = left ? 1.0f * w1 + 0.0f * w2 : 0.25f * w0 + 0.75f * w1;
+        h[r][1] = 0.75f * w1 + 0.25f * w2;
+        h[r][2] = 0.25f * w1 + 0.75f * w2;
+        h[r][3] = 0.75f * w2 + 0.25f * w3;
+    }
+    float *d = dst + (size_t)blockIdx.y * dst_stride;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const float w0 = win[pl][r][0], w1 = win[pl][r][1], w2 = win[pl][r][2], w3 = win[pl][r][3];
-            h[r][0] = left ? 1.0f * w1 + 0.0f * w2 : 0.25f * w0 + 0.75f * w1;
-            h[r][1] = 0.75f * w1 + 0.25f * w2;
-            h[r][2] = 0.25f * w1 + 0.75f * w2;
-            h[r][3] = 0.75f * w2 + 0.25f * w3;
-        }
+    for (int r = 0; r < kUpRows; ++r) {
+        const unsigned i = i0 + (unsigned)r;
+        if (i > last) break;
+        const bool top = i == 0u;
         float a[4], b[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            a[k] = top ? 1.0f * h[1][k] + 0.0f * h[2][k] : 0.25f * h[0][k] + 0.75f * h[1][k];
-            b[k] = 0.75f * h[1][k] + 0.25f * h[2][k];
+            a[k] = top ? 1.0f * h[r + 1][k] + 0.0f * h[r + 2][k] : 0.25f * h[r][k] + 0.75f * h[r + 1][k];
+            b[k] = 0.75f * h[r + 1][k] + 0.25f * h[r + 2][k];
         }
-        float *d = dst + (size_t)(p0 + pl) * dst_stride;
         const size_t o = (size_t)(2u * i) * (2u * res) + 4u * q;
         *reinterpret_cast<float4 *>(d + o) = make_float4(a[0], a[1], a[2], a[3]);
         *reinterpret_cast<float4 *>(d + o + 2u * res) = make_float4(b[0], b[1], b[2], b[3]);
@@ -653,10 +651,9 @@ int svbrdf_upsample2x_bilinear(const float *src, int res, size_t src_stride, flo
     if (s0 < d1 && d0 < s1) return fail(SVBRDF_EINVAL, "%s: source and destination overlap", what);
     hipStream_t s = (hipStream_t)stream;
     if (res % 2 == 0 && dst_stride % 4 == 0 && aligned(dst, 16) && src_stride % 2 == 0 && aligned(src, 8)) {
-        const unsigned threads = (unsigned)res * (unsigned)(res / 2);
-        const unsigned groups = (unsigned)((n_planes + kUpPlanes - 1) / kUpPlanes);
-        hipLaunchKernelGGL(upsample2x_x4_kernel, dim3(blocks_for(threads), groups), dim3(kBlock), 0, s, src,
-                           (unsigned)res, src_stride, dst, dst_stride, n_planes);
+        const unsigned threads = (unsigned)((res + kUpRows - 1) / kUpRows) * (unsigned)(res / 2);
+        hipLaunchKernelGGL(upsample2x_rows_kernel, dim3(blocks_for(threads), (unsigned)n_planes), dim3(kBlock), 0, s,
+                           src, (unsigned)res, src_stride, dst, dst_stride);
     } else {
         hipLaunchKernelGGL(upsample2x_kernel, dim3(blocks_for(n_out)), dim3(kBlock), 0, s, src, (unsigned)res,
                            src_stride, dst, dst_stride, n_planes);

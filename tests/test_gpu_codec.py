@@ -226,9 +226,10 @@ def test_upsample2x_padded_planes_and_validation(pkg):
 
 @pytest.mark.parametrize("res", [2, 4, 6, 96, 512])
 def test_upsample2x_vector_path_equals_scalar_path(pkg, res):
-    """Even resolutions with aligned destination rows take the 2 x 4 outputs
-    per thread kernel; a destination one float off alignment takes the one
-    output per thread kernel.  Both evaluate the same expressions, so they
+    """Even resolutions with aligned destination rows take the strip kernel
+    (4 source rows x 2 columns per thread; res 2 and 6 end in a partial
+    strip); a destination one float off alignment takes the one output per
+    thread kernel.  Both evaluate the same expressions, so they
     agree bit for bit, NaN and infinity propagation included (a NaN in the
     first column or row passes through the weight-0 neighbour term, as in
     torch)."""
