@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 6
+#define SVBRDF_ABI_VERSION 7
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -292,7 +292,8 @@ typedef struct svbrdf_material {
     const float *targets_prepared;  /* [V,3,R,R] from svbrdf_prepare_targets        */
     const float *camera_pos;        /* [V,3] cm                                     */
     const float *light_pos;         /* [V,3] cm                                     */
-    const float *light_pow;         /* [3]                                          */
+    float *light_pow;               /* [3]; updated in place when the batch
+                                       optimises the light power             */
     int res;                        /* R                                            */
     int n_views;                    /* V                                            */
     float size;                     /* plane width, cm                              */
@@ -303,6 +304,11 @@ typedef struct svbrdf_material {
     unsigned int texture_stride;    /* floats between the planes of textures;
                                        0 = R*R                                   */
     float loss_scale;               /* set by svbrdf_batch_plan                   */
+    float *light_adam;              /* [6]: Adam's exp_avg[3], exp_avg_sq[3] of
+                                       light_pow, zero before the first step;
+                                       read when the batch optimises the light
+                                       power (NULL: this material's light power
+                                       stays fixed)                          */
 } svbrdf_material;
 
 /* Number of tiles of `tile_pixels` pixels (a multiple of 256) the batch splits
@@ -320,9 +326,10 @@ int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixe
 size_t svbrdf_batch_workspace_bytes(size_t n_tiles);
 
 /* Number of blocks that fit on the device at once for a kernel variant
- * (pixels_per_thread in {1, 2}, views_per_trip in {1, 3}): the grid size of a
- * persistent launch is blocks_per_cu x compute units. */
-int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int views_per_trip);
+ * (pixels_per_thread in {1, 2}, views_per_trip in {1, 3}, with or without the
+ * light-power gradient): the grid size of a persistent launch is
+ * blocks_per_cu x compute units. */
+int svbrdf_batch_max_blocks_per_cu(int loss_mode, int optim_light, int pixels_per_thread, int views_per_trip);
 
 /* One optimisation iteration of every material of the batch (device copies of
  * the table and tile arrays).  losses[m] receives material m's loss;
@@ -330,11 +337,14 @@ int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int vie
  * the batch (all materials step together).  grid_blocks > 0 launches that
  * many persistent blocks drawing tiles from the ticket counter; grid_blocks
  * = 0 launches one block per tile instead (tile_pixels = 256, one pixel per
- * thread), in cost order, and leaves the scheduling to the dispatcher. */
+ * thread), in cost order, and leaves the scheduling to the dispatcher.
+ * optim_light != 0 also takes each material's Adam step on its light power
+ * (SvbrdfOptim.optim(optim_light=True), src/svbrdf.py:48-50,58-59, per
+ * material), with the moments in its light_adam. */
 int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const unsigned int *order,
                       const unsigned int *tile_material, size_t n_tiles, int tile_pixels,
                       int pixels_per_thread, int views_per_trip, int grid_blocks, int loss_mode,
-                      double lr, double beta1, double beta2, double eps, void *state, float *losses,
+                      int optim_light, double lr, double beta1, double beta2, double eps, void *state, float *losses,
                       float *loss_history, int history_len, void *workspace, size_t workspace_bytes,
                       void *stream);
 

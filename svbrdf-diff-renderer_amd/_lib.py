@@ -15,7 +15,7 @@ _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("SVBRDF_AMD_LIB") or _PKG / "libsvbrdf_hip.so")
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 OK = 0
 EINVAL = -1
@@ -74,9 +74,9 @@ SIGNATURES = {
     "svbrdf_batch_tile_count": (ctypes.c_size_t, [_vp, ctypes.c_int, ctypes.c_int]),
     "svbrdf_batch_plan": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_size_t]),
     "svbrdf_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
-    "svbrdf_batch_max_blocks_per_cu": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "svbrdf_batch_max_blocks_per_cu": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "svbrdf_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp,
                                          ctypes.c_int, _vp, ctypes.c_size_t, _vp]),
     # convolutions of the MaterialGAN caller's networks

@@ -37,7 +37,7 @@ class Material(ctypes.Structure):
                 ("light_pos", ctypes.c_void_p), ("light_pow", ctypes.c_void_p), ("res", ctypes.c_int),
                 ("n_views", ctypes.c_int), ("size", ctypes.c_float), ("tile_begin", ctypes.c_uint),
                 ("n_tiles", ctypes.c_uint), ("plane_stride", ctypes.c_uint), ("texture_stride", ctypes.c_uint),
-                ("loss_scale", ctypes.c_float)]
+                ("loss_scale", ctypes.c_float), ("light_adam", ctypes.c_void_p)]
 
 
 def _cache_path():
@@ -57,11 +57,15 @@ class BatchedStep:
     CUDA tensor of 9*R*R values (updated in place, or with own_textures=True
     copied into a buffer at the padded plane stride that write_back()
     returns) and targets [V,3,R,R].  All materials share lr, betas, eps and
-    the loss, and step together.
+    the loss, and step together.  With optim_light=True each material's light
+    power is optimised too (SvbrdfOptim.optim(optim_light=True) per material,
+    src/svbrdf.py:48-50,58-59): the renderer's own light_pow tensor (a
+    contiguous float32 [3] CUDA tensor) is updated in place, as FusedStep
+    does, with its Adam moments in ``light_adam[m]``.
     """
 
     def __init__(self, problems, lr, loss="log_l2", betas=(0.9, 0.999), eps=1e-8, history=0, config=None,
-                 own_textures=False):
+                 own_textures=False, optim_light=False):
         if not problems:
             raise ValueError("empty batch")
         self.device = _lib.cuda_device(problems[0][0].device, "BatchedStep")
@@ -96,8 +100,20 @@ class BatchedStep:
             self.exp_avg.append(_planes(moments[0], tex.shape, ps))
             self.exp_avg_sq.append(_planes(moments[1], tex.shape, ps))
             self.targets.append(prepared)
-        self.light_pow = [r.light_pow.detach().to(self.device, torch.float32).reshape(3).contiguous()
-                          for r in self.renderers]
+        self.optim_light = bool(optim_light)
+        if self.optim_light:
+            for r in self.renderers:
+                lp = r.light_pow
+                if (lp.device != self.device or lp.dtype != torch.float32 or lp.numel() != 3
+                        or not lp.is_contiguous()):
+                    raise ValueError("optim_light needs each renderer's light_pow as a contiguous float32 [3] "
+                                     "tensor on the batch's device")
+            self.light_pow = [r.light_pow for r in self.renderers]
+        else:
+            self.light_pow = [r.light_pow.detach().to(self.device, torch.float32).reshape(3).contiguous()
+                              for r in self.renderers]
+        # Adam's moments of each material's light power: exp_avg[3], exp_avg_sq[3]
+        self.light_adam = torch.zeros((len(problems), 8), dtype=torch.float32, device=self.device)
         self.n = len(problems)
         self.own_textures = bool(own_textures)
         nstate = _lib.query("svbrdf_step_state_bytes")
@@ -115,7 +131,8 @@ class BatchedStep:
     def signature(self):
         shape = sorted((r.res, r.n_of_imgs) for r in self.renderers)
         h = hashlib.sha1(json.dumps(shape).encode()).hexdigest()[:16]
-        return f"{torch.cuda.get_device_name(self.device)}|loss{self.mode}|{h}"
+        light = "|light" if self.optim_light else ""
+        return f"{torch.cuda.get_device_name(self.device)}|loss{self.mode}{light}|{h}"
 
     def configure(self, cfg):
         """Plans tiles for ``cfg`` and builds the device tables."""
@@ -131,7 +148,7 @@ class BatchedStep:
                 raise ValueError("blocks_per_cu=-1 (one block per tile) needs tile_pixels=256, pixels_per_thread=1")
             cfg["blocks_per_cu"] = -1
         else:
-            max_bpc = _lib.query("svbrdf_batch_max_blocks_per_cu", self.mode, npix_t, unroll)
+            max_bpc = _lib.query("svbrdf_batch_max_blocks_per_cu", self.mode, int(self.optim_light), npix_t, unroll)
             if max_bpc < 1:
                 raise RuntimeError("svbrdf_batch_max_blocks_per_cu failed: "
                                    + _lib.load().svbrdf_last_error().decode())
@@ -143,7 +160,8 @@ class BatchedStep:
                                 self.exp_avg_sq[i].data_ptr(), self.targets[i].data_ptr(),
                                 r.camera_pos.data_ptr(), r.light_pos.data_ptr(), self.light_pow[i].data_ptr(),
                                 r.res, r.n_of_imgs, r.size, 0, 0, self.plane_strides[i],
-                                self.plane_strides[i] if self.own_textures else 0)
+                                self.plane_strides[i] if self.own_textures else 0, 0.0,
+                                self.light_adam[i].data_ptr() if self.optim_light else None)
         n_tiles = _lib.query("svbrdf_batch_tile_count", table, self.n, tile)
         if n_tiles == 0:
             raise ValueError("invalid batch for tile planning")
@@ -171,7 +189,7 @@ class BatchedStep:
         hist = self.history if self.history.numel() else None
         _lib.call("svbrdf_batch_step", _lib.ptr(self.table), self.n, _lib.ptr(self.order), _lib.ptr(self.tile_mat),
                   self.n_tiles, c["tile_pixels"], c["pixels_per_thread"], c["views_per_trip"], self.grid,
-                  self.mode, self.lr, self.beta1, self.beta2, self.eps, _lib.ptr(self.state),
+                  self.mode, int(self.optim_light), self.lr, self.beta1, self.beta2, self.eps, _lib.ptr(self.state),
                   _lib.ptr(self.losses_dev), _lib.ptr(hist), self.history.shape[0], ctypes.c_void_p(self.ws_ptr),
                   self.ws_bytes, stream)
 
@@ -225,7 +243,8 @@ class BatchedStep:
     def _snapshot(self):
         return ([t.clone() for t in self.textures], [m.clone() for m in self.exp_avg],
                 [v.clone() for v in self.exp_avg_sq], self.state.clone(), self.losses_dev.clone(),
-                self.history.clone(), self.steps_launched)
+                self.history.clone(), self.steps_launched, [p.clone() for p in self.light_pow],
+                self.light_adam.clone())
 
     def _restore(self, snap):
         with torch.no_grad():
@@ -238,6 +257,9 @@ class BatchedStep:
             self.state.copy_(snap[3])
             self.losses_dev.copy_(snap[4])
             self.history.copy_(snap[5])
+            for dst, src in zip(self.light_pow, snap[7]):
+                dst.copy_(src)
+            self.light_adam.copy_(snap[8])
         self.steps_launched = snap[6]
 
     @staticmethod

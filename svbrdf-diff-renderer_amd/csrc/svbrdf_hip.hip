@@ -979,7 +979,7 @@ __global__ __launch_bounds__(kBlock) void flush_step_kernel(const void *ws, unsi
 #ifndef SVBRDF_BATCH_WAVES
 #define SVBRDF_BATCH_WAVES 4      // minimum waves per SIMD for the batched kernel's register allocation (128 VGPRs, no spills at one pixel per thread)
 #endif
-template <int LOSS, int NPIX, int UNROLL>
+template <int LOSS, int NPIX, int UNROLL, bool OPT_LIGHT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 1 ? SVBRDF_BATCH_WAVES : 1))) void batch_step_kernel(
     const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
     const unsigned *__restrict__ tile_mat, unsigned n_tiles, int tile_pixels, unsigned *__restrict__ queue,
@@ -1012,19 +1012,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
             for (int q = 0; q < NPIX; ++q)
                 px[q] = pixel_at(first + base + q * kBlock + threadIdx.x, npix, mt.res, mt.size);
             if (coloc)
-                fused_pixels<LOSS, false, true, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
-                                                              mt.targets_prepared, tps, ps, mt.n_views, mt.camera_pos,
-                                                              mt.light_pos, P, K2, sc, loss, gPs);
+                fused_pixels<LOSS, OPT_LIGHT, true, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
+                                                                  mt.targets_prepared, tps, ps, mt.n_views,
+                                                                  mt.camera_pos, mt.light_pos, P, K2, sc, loss, gPs);
             else
-                fused_pixels<LOSS, false, false, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
-                                                               mt.targets_prepared, tps, ps, mt.n_views, mt.camera_pos,
-                                                               mt.light_pos, P, K2, sc, loss, gPs);
+                fused_pixels<LOSS, OPT_LIGHT, false, NPIX, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
+                                                                   mt.targets_prepared, tps, ps, mt.n_views,
+                                                                   mt.camera_pos, mt.light_pos, P, K2, sc, loss, gPs);
         }
         // block_sum ends with a barrier, so every thread has read s_slot before
         // thread 0 overwrites it with the next ticket
         const float bl = block_sum(loss, lds);
+        float bp[3] = {0.f, 0.f, 0.f};
+        if (OPT_LIGHT) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bp[c] = block_sum(gPs[c], lds);
+        }
         if (threadIdx.x == 0) {
-            partials[id] = make_float4(bl, 0.f, 0.f, 0.f);
+            partials[id] = make_float4(bl, bp[0], bp[1], bp[2]);
             s_slot = atomicAdd(queue, 1u);
         }
         __syncthreads();
@@ -1037,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NPIX == 
 // costliest tiles start first), with one pixel per thread (256-pixel tiles).
 // No ticket counter and no loop; the register budget is the single-material
 // kernel's.
-template <int LOSS, int UNROLL>
+template <int LOSS, int UNROLL, bool OPT_LIGHT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_FUSED_WAVES))) void batch_grid_kernel(
     const svbrdf_material *__restrict__ mats, const unsigned *__restrict__ order,
     const unsigned *__restrict__ tile_mat, float4 *__restrict__ partials, StepConsts c,
@@ -1056,36 +1061,70 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_F
     Pix px[1];
     px[0] = pixel_at((id - mt.tile_begin) * (unsigned)kBlock + threadIdx.x, npix, mt.res, mt.size);
     if (all_colocated(mt.camera_pos, mt.light_pos, mt.n_views))
-        fused_pixels<LOSS, false, true, 1, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq, mt.targets_prepared,
-                                                   tps, ps, mt.n_views, mt.camera_pos, mt.light_pos, P, K2, sc, loss,
-                                                   gPs);
+        fused_pixels<LOSS, OPT_LIGHT, true, 1, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
+                                                       mt.targets_prepared, tps, ps, mt.n_views, mt.camera_pos,
+                                                       mt.light_pos, P, K2, sc, loss, gPs);
     else
-        fused_pixels<LOSS, false, false, 1, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq, mt.targets_prepared,
-                                                    tps, ps, mt.n_views, mt.camera_pos, mt.light_pos, P, K2, sc, loss,
-                                                    gPs);
+        fused_pixels<LOSS, OPT_LIGHT, false, 1, UNROLL>(px, mt.textures, mt.exp_avg, mt.exp_avg_sq,
+                                                        mt.targets_prepared, tps, ps, mt.n_views, mt.camera_pos,
+                                                        mt.light_pos, P, K2, sc, loss, gPs);
     const float bl = block_sum(loss, lds);
-    if (threadIdx.x == 0) partials[id] = make_float4(bl, 0.f, 0.f, 0.f);
+    float bp[3] = {0.f, 0.f, 0.f};
+    if (OPT_LIGHT) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bp[c] = block_sum(gPs[c], lds);
+    }
+    if (threadIdx.x == 0) partials[id] = make_float4(bl, bp[0], bp[1], bp[2]);
 }
 
 // Block m sums material m's tile partials in tile order (fixed, so the loss is
-// bit-reproducible) and records its loss; reads the step count only.
+// bit-reproducible) and records its loss; reads the step count only.  With
+// OPT_LIGHT it also sums the tiles' dL/dlight_pow and takes material m's
+// Adam step on its light power (one param group with the maps,
+// src/svbrdf.py:50: the maps' step scalars), moments in mt.light_adam.
+template <bool OPT_LIGHT>
 __global__ __launch_bounds__(kBlock) void batch_reduce_kernel(const svbrdf_material *__restrict__ mats,
-                                                              const float4 *__restrict__ partials,
+                                                              const float4 *__restrict__ partials, StepConsts k,
                                                               const svbrdf_step_state *__restrict__ st,
                                                               float *__restrict__ losses,
                                                               float *__restrict__ loss_history, int history_len,
                                                               int n_materials) {
+    constexpr int NQ = OPT_LIGHT ? 4 : 1;
     __shared__ double ldsd[kBlock / 64];
     const svbrdf_material mt = mats[blockIdx.x];
-    double acc = 0.0;
-    for (unsigned b = threadIdx.x; b < mt.n_tiles; b += kBlock) acc += (double)partials[mt.tile_begin + b].x;
-    const double tot = block_sum_d(acc, ldsd);
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (unsigned b = threadIdx.x; b < mt.n_tiles; b += kBlock) {
+        const float4 v = partials[mt.tile_begin + b];
+        acc[0] += (double)v.x;
+        if constexpr (OPT_LIGHT) {
+            acc[1] += (double)v.y;
+            acc[2] += (double)v.z;
+            acc[3] += (double)v.w;
+        }
+    }
+    double tot[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) tot[q] = block_sum_d(acc[q], ldsd);   // each ends with a barrier
     if (threadIdx.x != 0) return;
     const double count = (double)mt.n_views * 3.0 * (double)mt.res * (double)mt.res;
-    const float L = (float)(tot / count);
+    const float L = (float)(tot[0] / count);
     losses[blockIdx.x] = L;
     const int t = (int)st->step;   // completed steps before this one
     if (loss_history && t < history_len) loss_history[(size_t)t * n_materials + blockIdx.x] = L;
+    if constexpr (OPT_LIGHT) {
+        if (!mt.light_adam) return;
+        const StepScalars sc = step_scalars(st, k);
+        for (int c = 0; c < 3; ++c) {
+            float pp = mt.light_pow[c], mm = mt.light_adam[c], vv = mt.light_adam[3 + c];
+            adam_update(pp, mm, vv, (float)tot[1 + c], sc.w1, sc.beta2, sc.om_beta2, sc.inv_bc2_sqrt, sc.eps,
+                        sc.neg_step);
+            mt.light_pow[c] = pp;
+            mt.light_adam[c] = mm;
+            mt.light_adam[3 + c] = vv;
+        }
+    }
 }
 
 // Advances the shared step state and re-arms the ticket counter.
@@ -1101,10 +1140,32 @@ __global__ void batch_advance_kernel(svbrdf_step_state *__restrict__ st, unsigne
 
 size_t batch_partials_bytes(size_t n_tiles) { return (n_tiles * sizeof(float4) + 255) / 256 * 256; }
 
-template <int L>
+template <int L, bool LIGHT>
 const void *batch_kernel_ptr(int npix, int unroll) {
-    if (npix == 1) return unroll == 3 ? (const void *)batch_step_kernel<L, 1, 3> : (const void *)batch_step_kernel<L, 1, 1>;
-    return unroll == 3 ? (const void *)batch_step_kernel<L, 2, 3> : (const void *)batch_step_kernel<L, 2, 1>;
+    if (npix == 1)
+        return unroll == 3 ? (const void *)batch_step_kernel<L, 1, 3, LIGHT> : (const void *)batch_step_kernel<L, 1, 1, LIGHT>;
+    return unroll == 3 ? (const void *)batch_step_kernel<L, 2, 3, LIGHT> : (const void *)batch_step_kernel<L, 2, 1, LIGHT>;
+}
+
+template <int L, bool LIGHT>
+hipError_t launch_batch_step(const svbrdf_material *materials, const unsigned *order, const unsigned *tile_material,
+                       size_t n_tiles, int tile_pixels, int pixels_per_thread, int views_per_trip, int grid_blocks,
+                       unsigned *queue, float4 *partials, const StepConsts &consts, svbrdf_step_state *st,
+                       hipStream_t s) {
+    if (grid_blocks == 0) {
+        if (views_per_trip == 3)
+            hipLaunchKernelGGL((batch_grid_kernel<L, 3, LIGHT>), dim3((unsigned)n_tiles), dim3(kBlock), 0, s,
+                               materials, order, tile_material, partials, consts, st);
+        else
+            hipLaunchKernelGGL((batch_grid_kernel<L, 1, LIGHT>), dim3((unsigned)n_tiles), dim3(kBlock), 0, s,
+                               materials, order, tile_material, partials, consts, st);
+        return hipSuccess;
+    }
+    const void *k = batch_kernel_ptr<L, LIGHT>(pixels_per_thread, views_per_trip);
+    const unsigned nt = (unsigned)n_tiles;
+    void *args[] = {(void *)&materials, (void *)&order, (void *)&tile_material, (void *)&nt, (void *)&tile_pixels,
+                    (void *)&queue, (void *)&partials, (void *)&consts, (void *)&st};
+    return hipLaunchKernel(k, dim3(grid_blocks), dim3(kBlock), args, 0, s);
 }
 
 int check_variant(const char *what, int loss_mode, int npix, int unroll) {
@@ -1426,10 +1487,14 @@ int svbrdf_batch_plan(svbrdf_material *materials, int n_materials, int tile_pixe
 
 size_t svbrdf_batch_workspace_bytes(size_t n_tiles) { return batch_partials_bytes(n_tiles) + 256; }
 
-int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int views_per_trip) {
+int svbrdf_batch_max_blocks_per_cu(int loss_mode, int optim_light, int pixels_per_thread, int views_per_trip) {
     if (check_variant("svbrdf_batch_max_blocks_per_cu", loss_mode, pixels_per_thread, views_per_trip)) return 0;
-    const void *k = loss_mode == SVBRDF_LOSS_LOG_L2 ? batch_kernel_ptr<SVBRDF_LOSS_LOG_L2>(pixels_per_thread, views_per_trip)
-                                                    : batch_kernel_ptr<SVBRDF_LOSS_L2>(pixels_per_thread, views_per_trip);
+    const bool log_l2 = loss_mode == SVBRDF_LOSS_LOG_L2;
+    const void *k = optim_light
+                        ? (log_l2 ? batch_kernel_ptr<SVBRDF_LOSS_LOG_L2, true>(pixels_per_thread, views_per_trip)
+                                  : batch_kernel_ptr<SVBRDF_LOSS_L2, true>(pixels_per_thread, views_per_trip))
+                        : (log_l2 ? batch_kernel_ptr<SVBRDF_LOSS_LOG_L2, false>(pixels_per_thread, views_per_trip)
+                                  : batch_kernel_ptr<SVBRDF_LOSS_L2, false>(pixels_per_thread, views_per_trip));
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, 0) != hipSuccess) {
         (void)hipGetLastError();
@@ -1440,8 +1505,8 @@ int svbrdf_batch_max_blocks_per_cu(int loss_mode, int pixels_per_thread, int vie
 
 int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const unsigned int *order,
                       const unsigned int *tile_material, size_t n_tiles, int tile_pixels, int pixels_per_thread,
-                      int views_per_trip, int grid_blocks, int loss_mode, double lr, double beta1, double beta2,
-                      double eps, void *state, float *losses, float *loss_history, int history_len,
+                      int views_per_trip, int grid_blocks, int loss_mode, int optim_light, double lr, double beta1,
+                      double beta2, double eps, void *state, float *losses, float *loss_history, int history_len,
                       void *workspace, size_t workspace_bytes, void *stream) {
     const char *what = "svbrdf_batch_step";
     int rc = check_variant(what, loss_mode, pixels_per_thread, views_per_trip);
@@ -1468,39 +1533,23 @@ int svbrdf_batch_step(const svbrdf_material *materials, int n_materials, const u
     const StepConsts consts = make_consts(lr, beta1, beta2, eps, 0.0);   // K2 is per material
     unsigned *queue = (unsigned *)((char *)workspace + batch_partials_bytes(n_tiles));
     svbrdf_step_state *st = (svbrdf_step_state *)state;
-#define SVBRDF_LAUNCH_BATCH(L, N, U)                                                                          \
-    hipLaunchKernelGGL((batch_step_kernel<L, N, U>), dim3(grid_blocks), dim3(kBlock), 0, s, materials, order, \
-                       tile_material, (unsigned)n_tiles, tile_pixels, queue, partials, consts, st)
-#define SVBRDF_LAUNCH_BATCH_L(L)                                                                              \
-    do {                                                                                                       \
-        if (pixels_per_thread == 1) {                                                                          \
-            if (views_per_trip == 3) SVBRDF_LAUNCH_BATCH(L, 1, 3); else SVBRDF_LAUNCH_BATCH(L, 1, 1);          \
-        } else {                                                                                               \
-            if (views_per_trip == 3) SVBRDF_LAUNCH_BATCH(L, 2, 3); else SVBRDF_LAUNCH_BATCH(L, 2, 1);          \
-        }                                                                                                      \
-    } while (0)
-#define SVBRDF_LAUNCH_GRID(L, U)                                                                              \
-    hipLaunchKernelGGL((batch_grid_kernel<L, U>), dim3((unsigned)n_tiles), dim3(kBlock), 0, s, materials, order, \
-                       tile_material, partials, consts, st)
-    if (grid_blocks == 0) {
-        if (loss_mode == SVBRDF_LOSS_LOG_L2) {
-            if (views_per_trip == 3) SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_LOG_L2, 3);
-            else SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_LOG_L2, 1);
-        } else {
-            if (views_per_trip == 3) SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_L2, 3);
-            else SVBRDF_LAUNCH_GRID(SVBRDF_LOSS_L2, 1);
-        }
-    } else if (loss_mode == SVBRDF_LOSS_LOG_L2) {
-        SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_LOG_L2);
-    } else {
-        SVBRDF_LAUNCH_BATCH_L(SVBRDF_LOSS_L2);
+    const bool log_l2 = loss_mode == SVBRDF_LOSS_LOG_L2;
+    const auto launch = optim_light ? (log_l2 ? launch_batch_step<SVBRDF_LOSS_LOG_L2, true>
+                                              : launch_batch_step<SVBRDF_LOSS_L2, true>)
+                                    : (log_l2 ? launch_batch_step<SVBRDF_LOSS_LOG_L2, false>
+                                              : launch_batch_step<SVBRDF_LOSS_L2, false>);
+    if (launch(materials, order, tile_material, n_tiles, tile_pixels, pixels_per_thread, views_per_trip, grid_blocks,
+               queue, partials, consts, st, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SVBRDF_EHIP, "%s: step launch failed", what);
     }
-#undef SVBRDF_LAUNCH_GRID
-#undef SVBRDF_LAUNCH_BATCH_L
-#undef SVBRDF_LAUNCH_BATCH
     if ((rc = check_launch(what))) return rc;
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(n_materials), dim3(kBlock), 0, s, materials, partials, st, losses,
-                       loss_history, history_len, n_materials);
+    if (optim_light)
+        hipLaunchKernelGGL(batch_reduce_kernel<true>, dim3(n_materials), dim3(kBlock), 0, s, materials, partials,
+                           consts, st, losses, loss_history, history_len, n_materials);
+    else
+        hipLaunchKernelGGL(batch_reduce_kernel<false>, dim3(n_materials), dim3(kBlock), 0, s, materials, partials,
+                           consts, st, losses, loss_history, history_len, n_materials);
     if ((rc = check_launch(what))) return rc;
     hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, s, st, queue, beta1, beta2);
     return check_launch(what);

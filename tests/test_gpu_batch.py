@@ -254,3 +254,76 @@ def test_uneven_batch_against_the_reference(pkg, loss, mode):
                                  "blocks_per_cu": 1}))
     results = {name: first_step_in_batch(pkg, problems, cfg, loss) for name, cfg in forms}
     check_batch_against_eager(pkg, specs, problems, results, loss, mode, "uneven batch")
+
+
+def eager_light_grad(tex, target, res, size, cam, light, P, mode, dtype):
+    """dL/dlight_pow of the reference loop's loss through the eager
+    restatement (torch autograd on the CPU)."""
+    p = torch.tensor(P, dtype=dtype, requires_grad=True)
+    r = MT.MicrofacetEager(res, cam.shape[0], size, [torch.tensor(cam), torch.tensor(light), p], "cpu", dtype)
+    t = torch.as_tensor(tex).to(dtype)
+    L = MT.image_loss(r.eval(t[None].clamp(-1, 1)), torch.as_tensor(target).to(dtype), mode)
+    L.backward()
+    return p.grad.numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("cfg", [
+    {"tile_pixels": 256, "pixels_per_thread": 1, "views_per_trip": 1, "blocks_per_cu": -1},
+    {"tile_pixels": 4096, "pixels_per_thread": 1, "views_per_trip": 3, "blocks_per_cu": 0},
+    {"tile_pixels": 1024, "pixels_per_thread": 2, "views_per_trip": 1, "blocks_per_cu": 2},
+])
+@pytest.mark.parametrize("loss", ["log_l2", "l2"])
+def test_batch_light_matches_single_material_steps(pkg, cfg, loss):
+    """optim_light=True in the batch: every material's light power takes its
+    own Adam step (src/svbrdf.py:48-50,58-59 per material).  Three steps
+    against FusedStep(optim_light=True) run on each material alone (the
+    path the reference's golden light trajectories pin): light powers to 1e-5
+    relative (the batch sums dL/dP per tile, the single kernel per block, so
+    the two double sums may round one ulp apart), maps and second moments as
+    in test_batch_matches_single_material_steps, losses to rounding."""
+    specs = make_problems(pkg, SHAPES, seed=3)
+    batch, btex = build(pkg, specs, "batch", loss=loss, history=8, config=cfg, optim_light=True)
+    singles, stex = build(pkg, specs, "single", loss=loss, optim_light=True)
+    for _ in range(3):
+        batch.step()
+        for f in singles:
+            f.step()
+    torch.cuda.synchronize()
+    for i, f in enumerate(singles):
+        np.testing.assert_allclose(batch.light_pow[i].cpu().numpy(), f.light_pow.cpu().numpy(), rtol=1e-5,
+                                   err_msg=str((i, SHAPES[i])))
+        assert not np.array_equal(batch.light_pow[i].cpu().numpy(), specs[i][5]), "light power did not move"
+    for i, (b, s) in enumerate(zip(btex, stex)):
+        for x, y, rtol, atol in ((b, s, 1e-5, 1e-6), (batch.exp_avg_sq[i], singles[i].exp_avg_sq, 1e-4, 1e-14)):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            far = ~np.isclose(x, y, rtol=rtol, atol=atol)
+            info = (i, SHAPES[i], float(np.mean(x == y)), int(far.sum()), float(np.abs(x - y).max()))
+            assert np.mean(x == y) >= 0.99, info
+            assert far.sum() <= max(2, 1e-4 * x.size), info
+            assert np.abs(x - y).max() <= 3 * 2 * 0.01 * 0.1 / np.sqrt(1e-3) + 1e-6, info
+    hist = batch.loss_history()
+    for i, f in enumerate(singles):
+        np.testing.assert_allclose(hist[:, i], f.losses(), rtol=1e-5)
+
+
+@pytest.mark.parametrize("loss,mode", [("l2", M.LOSS_L2), ("log_l2", M.LOSS_LOG_L2)])
+def test_batch_light_first_step_against_the_reference(pkg, loss, mode):
+    """One batched step with optim_light=True: each material's dL/dlight_pow,
+    recovered exactly from its light Adam moment (m1 = 0.1 g), against torch
+    autograd of the eager restatement in FP32 and FP64 (within 1e-4 of FP64,
+    or no further than twice the reference FP32's own error), and the light
+    power after the step against Adam's first step from that gradient
+    (p1 = p0 - lr g / (|g| + eps))."""
+    specs = make_problems(pkg, SHAPES, seed=5)
+    batch, _ = build(pkg, specs, "batch", loss=loss, optim_light=True)
+    batch.step()
+    torch.cuda.synchronize()
+    adam = batch.light_adam.cpu().numpy().astype(np.float64)
+    for i, (res, V, size, cam, light, P, init, target) in enumerate(specs):
+        g = adam[i, :3] / np.float64(np.float32(0.1))
+        g64 = eager_light_grad(init, target, res, size, cam, light, P, mode, torch.float64)
+        g32 = eager_light_grad(init, target, res, size, cam, light, P, mode, torch.float32)
+        bound = np.maximum(1e-4 * np.abs(g64), 2 * np.abs(g32 - g64)) + 1e-30
+        assert np.all(np.abs(g - g64) <= bound), (i, SHAPES[i], g, g64, g32)
+        p1 = np.asarray(P, np.float64) - 0.01 * g / (np.abs(g) + 1e-8)
+        np.testing.assert_allclose(batch.light_pow[i].cpu().numpy(), p1, rtol=1e-6, err_msg=str(i))

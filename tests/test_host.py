@@ -100,7 +100,7 @@ def test_batch_step_one_block_per_tile_needs_256_pixel_tiles(pkg):
     ws = lib.svbrdf_batch_workspace_bytes(64)
 
     def step(tile, npix_t, grid):
-        return lib.svbrdf_batch_step(fake, 1, fake, fake, 64, tile, npix_t, 1, grid, 1, ctypes.c_double(0.01),
+        return lib.svbrdf_batch_step(fake, 1, fake, fake, 64, tile, npix_t, 1, grid, 1, 0, ctypes.c_double(0.01),
                                      ctypes.c_double(0.9), ctypes.c_double(0.999), ctypes.c_double(1e-8),
                                      fake, fake, None, 0, fake, ws, None)
     assert step(1024, 1, 0) == -1
