@@ -662,8 +662,9 @@ def compose_leg(pkg, dev, res=256, views=9, iters=100):
 def conv_layers_leg(pkg, dev, reps=20):
     """Every 3x3 layer shape of the two networks at the legs' sizes -- the
     VGG-19-shaped stack on 9 images of 256^2 (forward and input-gradient
-    shapes) and the 256^2 generator at batch 1 -- on the HIP convolution
-    (svbrdf_conv3x3, bias + ReLU epilogue) and on F.conv2d + ReLU (MIOpen),
+    shapes) and the 256^2 generator at batch 1 (plus its weight gradient) --
+    on the HIP kernels (svbrdf_conv3x3 with a bias + ReLU epilogue,
+    svbrdf_conv3x3_wgrad) and on F.conv2d + ReLU / conv2d_weight (MIOpen),
     HIP events around `reps` launches after 3 warm-ups; TFLOP/s at
     2 * 9 * Cin * Cout * pixels."""
     from svbrdf_amd import convnet
@@ -676,7 +677,7 @@ def conv_layers_leg(pkg, dev, reps=20):
                          (512, 256, 128), (256, 256, 128), (256, 128, 256), (128, 128, 256)]:
         shapes.append(("generator", 1, cin, cout, r))
     g = torch.Generator().manual_seed(11)
-    rows, tot_hip, tot_lib = [], 0.0, 0.0
+    rows, tot_hip, tot_lib, tot_whip, tot_wlib = [], 0.0, 0.0, 0.0, 0.0
     for net, n, cin, cout, r in shapes:
         x = torch.randn((n, cin, r, r), generator=g).to(dev)
         w = (torch.randn((cout, cin, 3, 3), generator=g) / (cin * 9) ** 0.5).to(dev)
@@ -696,12 +697,34 @@ def conv_layers_leg(pkg, dev, reps=20):
         flops = 2.0 * 9 * cin * cout * n * r * r
         tot_hip += t_hip
         tot_lib += t_lib
-        rows.append({"net": net, "n": n, "cin": cin, "cout": cout, "side": r, "hip_us": round(t_hip, 1),
-                     "hip_tflops": round(flops / t_hip / 1e6, 1), "miopen_us": round(t_lib, 1),
-                     "miopen_tflops": round(flops / t_lib / 1e6, 1)})
+        row = {"net": net, "n": n, "cin": cin, "cout": cout, "side": r, "hip_us": round(t_hip, 1),
+               "hip_tflops": round(flops / t_hip / 1e6, 1), "miopen_us": round(t_lib, 1),
+               "miopen_tflops": round(flops / t_lib / 1e6, 1)}
+        if net == "generator":
+            # the modulated weights' gradient (latent epochs): svbrdf_conv3x3_wgrad
+            # against torch.nn.grad.conv2d_weight
+            gz = torch.randn((n, cout, r, r), generator=g).to(dev)
+
+            def hip_w():
+                convnet.conv3x3_wgrad(x, gz, cout)
+
+            def lib_w():
+                torch.nn.grad.conv2d_weight(x, (cout, cin, 3, 3), gz, padding=1)
+            for fn in (hip_w, lib_w):
+                for _ in range(3):
+                    fn()
+            tw_hip = time_events(hip_w, reps) * 1e3
+            tw_lib = time_events(lib_w, reps) * 1e3
+            tot_whip += tw_hip
+            tot_wlib += tw_lib
+            row.update(hip_wgrad_us=round(tw_hip, 1), hip_wgrad_tflops=round(flops / tw_hip / 1e6, 1),
+                       miopen_wgrad_us=round(tw_lib, 1), miopen_wgrad_tflops=round(flops / tw_lib / 1e6, 1))
+        rows.append(row)
     return {"workload": "3x3 layers of the VGG-19-shaped stack (9 x 256^2, forward and dL/dx shapes) and the 256^2 "
                         "StyleGAN2-shaped generator (batch 1), fp32; HIP vs F.conv2d (MIOpen)",
-            "total_hip_us": round(tot_hip, 1), "total_miopen_us": round(tot_lib, 1), "layers": rows}
+            "total_hip_us": round(tot_hip, 1), "total_miopen_us": round(tot_lib, 1),
+            "generator_wgrad_total_hip_us": round(tot_whip, 1), "generator_wgrad_total_miopen_us": round(tot_wlib, 1),
+            "layers": rows}
 
 
 def host_cpu():

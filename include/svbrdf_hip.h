@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 7
+#define SVBRDF_ABI_VERSION 8
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -402,6 +402,20 @@ int svbrdf_conv3x3(const float *x, int n, int cin, int height, int width, const 
 int svbrdf_conv3x3_input_grad(const float *grad_y, const float *y, int activation, int n, int cout, int height,
                               int width, const float *packed_t, int cin, float *grad_x, void *workspace,
                               size_t workspace_bytes, void *stream);
+
+/* dL/dw [cout, cin, 3, 3] of a 3x3 convolution (stride 1, zero padding 1)
+ * from its input x [n, cin, height, width] and grad_z [n, cout, height,
+ * width], the gradient of the loss with respect to the convolution's output
+ * before bias and activation (torch.nn.grad.conv2d_weight, the weight
+ * gradient torch autograd takes through F.conv2d; for the modulated weights
+ * of ModulateConvBlock, stylegan2_generator_network.py:694-755, it then flows
+ * to the style and the latent).  Overwrites grad_w.  Many pixels per channel
+ * tile split the pixel sum into slices: workspace of
+ * svbrdf_conv3x3_wgrad_workspace_bytes(n, cin, height, width, cout) bytes (0:
+ * workspace may be NULL); the slices are added in a fixed order. */
+size_t svbrdf_conv3x3_wgrad_workspace_bytes(int n, int cin, int height, int width, int cout);
+int svbrdf_conv3x3_wgrad(const float *x, const float *grad_z, int n, int cin, int height, int width, int cout,
+                         float *grad_w, void *workspace, size_t workspace_bytes, void *stream);
 
 /* x2 upsampling of `planes` planes of height x width: zero insertion and the
  * [1, 3, 3, 1] binomial filter with gain 4 (UpsamplingLayer,

@@ -10,7 +10,8 @@ kernels of libsvbrdf_hip.so (csrc/conv.hip).
   dL/dx are the HIP kernel (dL/dx is the convolution of dL/dy with the
   flipped, transposed weights; where only dL/dx is wanted the kernel applies
   the activation's derivative to dL/dy itself, :func:`conv3x3_input_grad`),
-  dL/dweights is torch's weight-gradient convolution.
+  and so is dL/dweights (:func:`conv3x3_wgrad`, an MFMA GEMM over the pixels
+  with the nine taps sharing the dL/dy operand).
 * :func:`upsample2` -- x2 upsampling by zero insertion and the [1, 3, 3, 1]
   filter (UpsamplingLayer, stylegan2_generator_network.py:419-479); forward
   and adjoint are HIP kernels.
@@ -96,6 +97,25 @@ def conv3x3_input_grad(grad_y, y, packed_t, cin, act):
     return gx
 
 
+def conv3x3_wgrad(x, grad_z, cout):
+    """dL/dw [cout, Cin, 3, 3] of a 3x3 convolution (padding 1) from its input
+    x [N, Cin, H, W] and grad_z [N, cout, H, W], the gradient with respect to
+    its output before bias and activation (torch.nn.grad.conv2d_weight)."""
+    x = x.contiguous()
+    g = grad_z.contiguous()
+    _check(x, "x")
+    _check(g, "grad_z")
+    n, cin, h, w = (int(s) for s in x.shape)
+    if tuple(g.shape) != (n, cout, h, w):
+        raise ValueError(f"grad_z {tuple(g.shape)} does not match x {tuple(x.shape)} and Cout={cout}")
+    gw = torch.empty((cout, cin, 3, 3), dtype=torch.float32, device=x.device)
+    ws_bytes = int(_lib.query("svbrdf_conv3x3_wgrad_workspace_bytes", n, cin, h, w, cout))
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=x.device) if ws_bytes else None
+    _lib.call("svbrdf_conv3x3_wgrad", _lib.ptr(x), _lib.ptr(g), n, cin, h, w, cout, _lib.ptr(gw), _lib.ptr(ws),
+              ws_bytes, _lib.stream_handle(x.device))
+    return gw
+
+
 def _act_grad(grad_y, y, act):
     """dL/d(pre-activation) from dL/dy and y: the activations keep the sign."""
     if act == "relu":
@@ -130,7 +150,7 @@ class Conv3x3Function(torch.autograd.Function):
         if ctx.needs_input_grad[0]:
             gx = conv3x3_packed(gz, pack3x3(weight, transpose=True), int(weight.shape[1]))
         if ctx.needs_input_grad[1]:
-            gw = torch.nn.grad.conv2d_weight(x, weight.shape, gz, padding=1)
+            gw = conv3x3_wgrad(x, gz, int(weight.shape[0]))
         if ctx.needs_input_grad[3] and noise is not None:
             gnoise = (ctx.noise_strength * gz.sum(dim=(0, 1))).reshape(noise.shape)
         return gx, gw, None, gnoise, None, None

@@ -22,6 +22,10 @@
 //                         partial sums go to the caller's workspace;
 //   conv_split_sum_kernel then adds the slices in order and applies the
 //                         epilogue.
+//   conv3x3_wgrad_kernel  dL/dw of the convolution: per tap a GEMM over the
+//                         pixels (M = Cout, N = Cin), the nine taps sharing
+//                         the dL/dy operand; wgrad_sum_kernel adds its
+//                         pixel slices.
 //   conv3x3_direct_kernel the same convolution for at most 8 output channels
 //                         (the input gradient of a layer that reads the RGB
 //                         image): two pixels per thread, all output channels
@@ -391,6 +395,153 @@ void launch_direct(const ConvArgs &a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// Weight gradient: dL/dw[co][ci][t] = sum over images and pixels p of
+// gz[co][p] x[ci][p + shift(t)] (gz = dL/d(pre-activation), zero padding).
+// Per tap a GEMM with M = output channels, N = input channels, K = pixels;
+// the nine taps share the gz operand.  A block owns 64 x 64 (co, ci) for all
+// nine taps (each wave a 32 x 32 quarter: 9 MFMA accumulator tiles), and K
+// runs over chunks of 16 pixels (an 8 x 2 tile of one image, 4 x 4 for
+// narrower images): the chunk's gz
+// values and its input tile with a one-pixel halo go to LDS transposed
+// (pixel-major, channels contiguous; a row pitch of 97 = 33 mod 64 floats
+// keeps the two half-waves' rows, one pixel apart, on disjoint banks and the
+// stores of consecutive pixels on distinct ones), double-buffered with the next chunk's loads in flight.  Many
+// pixels and few channel tiles: the chunks are split into slices whose
+// partial sums a second kernel adds in slice order.
+constexpr int kWgM = 64, kWgN = 64, kWgP = 16, kWgPitch = 97;
+
+struct WgradArgs {
+    const float *x;    // [N, Cin, H, W]
+    const float *gz;   // [N, Cout, H, W]
+    float *out;        // [Cout][Cin][9], or [slices][Cout][Cin][9] partial sums
+    int n, cin, cout, h, w_;
+    int tiles_x, tiles_per_img, n_chunks, cps;
+};
+
+// Two waves per SIMD: the nine 32 x 32 accumulator tiles take 144 of a
+// wave's 256 registers.
+template <int TW>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void conv3x3_wgrad_kernel(WgradArgs a) {
+    constexpr int TH = kWgP / TW, TW2 = TW + 2, HS = (TH + 2) * TW2;   // the chunk's pixel tile and halo
+    constexpr int G_PER = kWgM * kWgP / kThreads;
+    constexpr int X_PER = (kWgN * HS + kThreads - 1) / kThreads;
+    __shared__ float Gs[2][kWgP * kWgPitch];
+    __shared__ float Xs[2][HS * kWgPitch];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;   // the wave's 32 output and 32 input channels
+    const int co0 = blockIdx.x * kWgM, ci0 = blockIdx.y * kWgN;
+    const int slice = blockIdx.z;
+    const int H = a.h, W = a.w_, HW = H * W;
+    const int c_begin = slice * a.cps, c_end = min(a.n_chunks, c_begin + a.cps);
+
+    float greg[G_PER], xreg[X_PER];
+    auto load_chunk = [&](int c) {
+        const int img = c / a.tiles_per_img, tile = c - img * a.tiles_per_img;
+        const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+        const int y0 = ty * TH, x0 = tx * TW;
+        const float *gi = a.gz + (size_t)img * a.cout * HW;
+        const float *xi = a.x + (size_t)img * a.cin * HW;
+#pragma unroll
+        for (int k = 0; k < G_PER; ++k) {
+            const int q = tid + k * kThreads;
+            const int co = q / kWgP, pp = q - co * kWgP;
+            const int yy = y0 + pp / TW, xx = x0 + pp % TW;
+            const bool ok = co0 + co < a.cout && yy < H && xx < W;
+            const float v = gi[ok ? (size_t)(co0 + co) * HW + yy * W + xx : 0];
+            greg[k] = ok ? v : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < X_PER; ++k) {
+            const int q = tid + k * kThreads;
+            const int ci = q / HS, hp = q - ci * HS;
+            const int yy = y0 - 1 + hp / TW2, xx = x0 - 1 + hp % TW2;
+            const bool ok = q < kWgN * HS && ci0 + ci < a.cin && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            const float v = xi[ok ? (size_t)(ci0 + ci) * HW + yy * W + xx : 0];
+            xreg[k] = ok ? v : 0.0f;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < G_PER; ++k) {
+            const int q = tid + k * kThreads;
+            const int co = q / kWgP, pp = q - co * kWgP;
+            Gs[buf][pp * kWgPitch + co] = greg[k];
+        }
+#pragma unroll
+        for (int k = 0; k < X_PER; ++k) {
+            const int q = tid + k * kThreads;
+            if (q < kWgN * HS) {
+                const int ci = q / HS, hp = q - ci * HS;
+                Xs[buf][hp * kWgPitch + ci] = xreg[k];
+            }
+        }
+    };
+
+    v16f acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+    if (c_begin < c_end) {
+        load_chunk(c_begin);
+        store_chunk(0);
+    }
+    __syncthreads();
+    const int kp = lane >> 5;       // which pixel of a K-step pair this lane carries
+    for (int c = c_begin; c < c_end; ++c) {
+        const int buf = (c - c_begin) & 1;
+        if (c + 1 < c_end) load_chunk(c + 1);
+        const float *G = Gs[buf] + wm * 32 + (lane & 31);
+        const float *X = Xs[buf] + wn * 32 + (lane & 31);
+#pragma unroll 2
+        for (int ks = 0; ks < kWgP / 2; ++ks) {
+            const int pp = 2 * ks + kp;
+            const int py = pp / TW, px = pp - py * TW;
+            const float av = G[pp * kWgPitch];
+            const float *xb = X + (py * TW2 + px) * kWgPitch;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const float bv = xb[((t / 3) * TW2 + (t % 3)) * kWgPitch];
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+        if (c + 1 < c_end) store_chunk(buf ^ 1);
+        __syncthreads();
+    }
+
+    // element r of tile t: co row 8 (r / 4) + 4 (lane / 32) + r % 4, ci column lane % 32
+    float *o = a.out + (size_t)slice * a.cout * a.cin * 9;
+    const int ci = ci0 + wn * 32 + (lane & 31);
+    if (ci >= a.cin) return;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int co = co0 + wm * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (co >= a.cout) continue;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) o[((size_t)co * a.cin + ci) * 9 + t] = acc[t][r];
+    }
+}
+
+// Sum of the weight-gradient slices: eight running sums over the slices
+// s = j mod 8 (eight loads in flight per trip), then combined in a fixed
+// order -- the same bits on every run.
+__global__ __launch_bounds__(kThreads) void wgrad_sum_kernel(const float *__restrict__ part, int slices, size_t total,
+                                                             float *__restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (e >= total) return;
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = 0.0f;
+    int s = 0;
+    for (; s + 8 <= slices; s += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] += part[(size_t)(s + j) * total + e];
+    }
+    for (int j = 0; s + j < slices; ++j) v[j] += part[(size_t)(s + j) * total + e];
+    out[e] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
+// ---------------------------------------------------------------------------
 // x2 upsampling with the [1, 3, 3, 1] filter and its adjoint.  Zero insertion
 // (x(p) at 2p), padding 2 before and 1 after, and the filter [1, 3, 3, 1]/4
 // per axis give, per axis,
@@ -540,6 +691,49 @@ ConvPlan plan_conv(int n, int cin, int h, int w, int cout) {
     return best;
 }
 
+// Tiling of the weight gradient: 64 x 64 (co, ci) blocks, pixel chunks of
+// 16 split into slices by the same cost reasoning as plan_conv (rounds of
+// blocks per compute unit x chunks per slice, plus the sum pass).
+struct WgradPlan {
+    int tw, th, tiles_x, tiles_per_img, n_chunks, co_tiles, ci_tiles, slices, cps;
+    size_t part_floats;
+};
+
+WgradPlan plan_wgrad(int n, int cin, int h, int w, int cout) {
+    WgradPlan p{};
+    p.tw = std::min(pick_tile_width(w), 8);   // 8 x 2 pixel tiles: the smallest halo, no spills
+    p.th = kWgP / p.tw;
+    p.tiles_x = (w + p.tw - 1) / p.tw;
+    p.tiles_per_img = p.tiles_x * ((h + p.th - 1) / p.th);
+    p.n_chunks = n * p.tiles_per_img;
+    p.co_tiles = (cout + kWgM - 1) / kWgM;
+    p.ci_tiles = (cin + kWgN - 1) / kWgN;
+    const long long blocks = (long long)p.co_tiles * p.ci_tiles;
+    const double chunk_us = 2.0 * kWgM * kWgN * 9 * kWgP / 0.45e6;
+    const double out_bytes = 9.0 * cin * cout * sizeof(float);
+    double best_t = 0.0;
+    p.slices = 1;
+    p.cps = p.n_chunks;
+    const int max_slices = std::max(1, std::min(256, p.n_chunks / kMinChunksPerSlice));
+    for (int s = 1; s <= max_slices; ++s) {
+        const int cps = (p.n_chunks + s - 1) / s;
+        if ((p.n_chunks + cps - 1) / cps != s) continue;   // no empty slice
+        const long long b = blocks * s;
+        double t = (double)((b + kComputeUnits - 1) / kComputeUnits) * (cps + 1) * chunk_us;
+        if (b <= kComputeUnits) t /= 0.85;
+        // the slices' partial sums: written by the blocks, read by the sum
+        // pass (measured ~5 TB/s at 150 MB)
+        if (s > 1) t += 2.0 * s * out_bytes / 5.0e6 + 2.0;
+        if (s == 1 || t < 0.98 * best_t) {
+            best_t = t;
+            p.slices = s;
+            p.cps = cps;
+        }
+    }
+    p.part_floats = p.slices > 1 ? (size_t)p.slices * 9 * cin * cout : 0;
+    return p;
+}
+
 template <int BM, int BN>
 void launch_mfma(const ConvArgs &a, const ConvPlan &p, hipStream_t s) {
     const dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)p.co_tiles, (unsigned)(a.n * p.slices));
@@ -642,6 +836,39 @@ int svbrdf_conv3x3_input_grad(const float *grad_y, const float *y, int activatio
         a.gate_neg = activation == SVBRDF_ACT_LRELU ? 0.2f * 1.41421356237309515f : 0.0f;
     }
     return run_conv(what, a, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+size_t svbrdf_conv3x3_wgrad_workspace_bytes(int n, int cin, int height, int width, int cout) {
+    if (!conv_shape_ok(n, cin, height, width, cout)) return 0;
+    return plan_wgrad(n, cin, height, width, cout).part_floats * sizeof(float);
+}
+
+int svbrdf_conv3x3_wgrad(const float *x, const float *grad_z, int n, int cin, int height, int width, int cout,
+                         float *grad_w, void *workspace, size_t workspace_bytes, void *stream) {
+    const char *what = "svbrdf_conv3x3_wgrad";
+    if (!x || !grad_z || !grad_w) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (!conv_shape_ok(n, cin, height, width, cout))
+        return fail(SVBRDF_ESHAPE, "%s: n=%d cin=%d cout=%d height=%d width=%d", what, n, cin, cout, height, width);
+    const WgradPlan p = plan_wgrad(n, cin, height, width, cout);
+    const size_t need = p.part_floats * sizeof(float);
+    if (need && (!workspace || workspace_bytes < need))
+        return fail(SVBRDF_ESPACE, "%s: workspace of %zu bytes, this shape needs %zu "
+                    "(svbrdf_conv3x3_wgrad_workspace_bytes)", what, workspace ? workspace_bytes : (size_t)0, need);
+    WgradArgs a{};
+    a.x = x; a.gz = grad_z; a.out = p.slices > 1 ? (float *)workspace : grad_w;
+    a.n = n; a.cin = cin; a.cout = cout; a.h = height; a.w_ = width;
+    a.tiles_x = p.tiles_x; a.tiles_per_img = p.tiles_per_img; a.n_chunks = p.n_chunks;
+    a.cps = p.cps;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)p.co_tiles, (unsigned)p.ci_tiles, (unsigned)p.slices);
+    if (p.tw == 8) hipLaunchKernelGGL(conv3x3_wgrad_kernel<8>, grid, dim3(kThreads), 0, s, a);
+    else hipLaunchKernelGGL(conv3x3_wgrad_kernel<4>, grid, dim3(kThreads), 0, s, a);
+    if (p.slices > 1) {
+        const size_t total = (size_t)9 * cin * cout;
+        hipLaunchKernelGGL(wgrad_sum_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           s, (const float *)workspace, p.slices, total, grad_w);
+    }
+    return check_launch(what);
 }
 
 int svbrdf_upsample2_blur(const float *x, int planes, int height, int width, float *y, void *stream) {

@@ -121,6 +121,42 @@ def test_conv3x3_input_grad_gated(pkg, act, shape):
     assert rel2 <= 1e-6, rel2
 
 
+WGRAD = [  # n, cin, cout, h, w
+    (2, 96, 160, 20, 28),
+    (1, 512, 512, 4, 4),
+    (1, 128, 128, 256, 256),
+    (3, 13, 70, 5, 7),
+    (1, 3, 64, 33, 17),
+    (4, 64, 3, 16, 16),
+    (1, 256, 512, 64, 64),
+]
+
+
+@pytest.mark.parametrize("shape", WGRAD, ids=[f"{s[0]}x{s[1]}->{s[2]}@{s[3]}x{s[4]}" for s in WGRAD])
+def test_conv3x3_weight_gradient(pkg, shape):
+    """svbrdf_conv3x3_wgrad against torch.nn.grad.conv2d_weight in FP64: every
+    element within 2e-5 of the FP64 value relative to the same sum over |x|
+    and |dL/dz| (the scale of its rounding error), plus 1e-7; the pixel-sum
+    slices of the large images are added in a fixed order (repeatable bits)."""
+    from svbrdf_amd import convnet
+    n, cin, cout, h, w = shape
+    g = torch.Generator().manual_seed(cin * 7 + cout)
+    x = torch.randn((n, cin, h, w), generator=g).to(DEV)
+    gz = torch.randn((n, cout, h, w), generator=g).to(DEV)
+    gw = convnet.conv3x3_wgrad(x, gz, cout)
+    assert torch.equal(gw, convnet.conv3x3_wgrad(x, gz, cout))
+    with torch.no_grad():
+        ref = torch.nn.grad.conv2d_weight(x.double(), (cout, cin, 3, 3), gz.double(), padding=1)
+        scale = torch.nn.grad.conv2d_weight(x.double().abs(), (cout, cin, 3, 3), gz.double().abs(), padding=1)
+        ref32 = torch.nn.grad.conv2d_weight(x, (cout, cin, 3, 3), gz, padding=1)
+    torch.cuda.synchronize()
+    err = (gw.double() - ref).abs()
+    ratio = float((err / (2e-5 * scale + 1e-7)).max())
+    e32 = float(((ref32.double() - ref).abs() / (scale + 1e-30)).max())
+    print(f"wgrad {shape}: max error / scale {float((err / (scale + 1e-30)).max()):.2e} (MIOpen FP32 {e32:.2e})")
+    assert ratio <= 1.0, (shape, ratio)
+
+
 def test_conv3x3_refuses_a_short_workspace(pkg):
     """A shape that splits the channel sum needs the queried workspace; one
     float short is refused before any launch."""
