@@ -148,7 +148,7 @@ __device__ __forceinline__ float finish(const ConvArgs &a, float v, int co, size
 // hidden by the chunk's 9 taps x KC/2 K-steps of MFMA work and one barrier
 // per chunk separates the buffers' uses.  VEC_A loads the weights as float4
 // (Cout a multiple of 4).
-template <int BM, int BN, int KC, bool VEC_A>
+template <int BM, int BN, int KC, bool VEC_A, bool GATE>
 __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
     constexpr int TM = BM / 64, TN = BN / 64;   // 32 x 32 MFMA tiles per wave along M and N
     constexpr int HF = halo_floats<BN>();
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
         const int lim = a.cin - ci0;            // channels left: the last chunk may be partial
         const float *wc = a.w + (size_t)ci0 * a.cout;
         const float *xc = xi + (size_t)ci0 * HW;
-        const float *gc = a.gate ? a.gate + (size_t)img * a.cin * HW + (size_t)ci0 * HW : nullptr;
+        const float *gc = GATE ? a.gate + (size_t)img * a.cin * HW + (size_t)ci0 * HW : nullptr;
 #pragma unroll
         for (int k = 0; k < A_PER; ++k) {
             aok[k] = aoff[k] >= 0 && akc[k] < lim;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
             bok[k] = boff[k] >= 0 && bkc[k] < lim;
             const int o = bok[k] ? boff[k] : 0;
             breg[k] = xc[o];
-            if (gc) greg[k] = gc[o];
+            if constexpr (GATE) greg[k] = gc[o];
         }
     };
     auto store_chunk = [&](int buf) {
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void conv3x3_mfma_kernel(ConvArgs a) {
         for (int k = 0; k < B_PER; ++k) {
             const int q = tid + k * kThreads;
             float v = breg[k];
-            if (a.gate) v = v * (greg[k] > 0.0f ? a.gate_pos : a.gate_neg);
+            if constexpr (GATE) v = v * (greg[k] > 0.0f ? a.gate_pos : a.gate_neg);
             if (bkc[k] < KC) Bs[buf][q + bkc[k] * (HF - hs)] = bok[k] ? v : 0.0f;   // row kc of the buffer
         }
     };
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void conv_split_sum_kernel(ConvArgs a) {
 // taps), the 2 x CO sums in registers.  The weights of one input channel and
 // tap are the same for every thread: scalar loads.  Out-of-image taps load a
 // clamped in-image address and select 0, so the loop has no branches.
-template <int CO>
+template <int CO, bool GATE>
 __global__ __launch_bounds__(kThreads) void conv3x3_direct_kernel(ConvArgs a) {
     const int H = a.h, W = a.w_;
     const int xx = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -354,17 +354,17 @@ __global__ __launch_bounds__(kThreads) void conv3x3_direct_kernel(ConvArgs a) {
 #pragma unroll
     for (int co = 0; co < CO; ++co) acc0[co] = acc1[co] = 0.0f;
     const float *xc = a.x + (size_t)img * a.cin * HW;
-    const float *gc = a.gate ? a.gate + (size_t)img * a.cin * HW : nullptr;
+    const float *gc = GATE ? a.gate + (size_t)img * a.cin * HW : nullptr;
     const size_t tap_stride = (size_t)a.cin * CO;
 #pragma unroll 2
-    for (int ci = 0; ci < a.cin; ++ci, xc += HW, gc = gc ? gc + HW : nullptr) {
+    for (int ci = 0; ci < a.cin; ++ci, xc += HW, gc += GATE ? HW : 0) {
         float v[4][3];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 float raw = xc[off[r][c]];
-                if (gc) raw = raw * (gc[off[r][c]] > 0.0f ? a.gate_pos : a.gate_neg);
+                if constexpr (GATE) raw = raw * (gc[off[r][c]] > 0.0f ? a.gate_pos : a.gate_neg);
                 v[r][c] = in[r][c] ? raw : 0.0f;
             }
         const float *wc = a.w + (size_t)ci * CO;
@@ -391,7 +391,10 @@ __global__ __launch_bounds__(kThreads) void conv3x3_direct_kernel(ConvArgs a) {
 template <int CO>
 void launch_direct(const ConvArgs &a, hipStream_t s) {
     const dim3 grid((unsigned)((a.w_ + 63) / 64), (unsigned)((a.h + 7) / 8), (unsigned)a.n);
-    hipLaunchKernelGGL(conv3x3_direct_kernel<CO>, grid, dim3(kThreads), 0, s, a);
+    // the gate is a separate instantiation: a null test inside the unrolled
+    // loop puts a branch (and a wait) between the loads
+    if (a.gate) hipLaunchKernelGGL((conv3x3_direct_kernel<CO, true>), grid, dim3(kThreads), 0, s, a);
+    else hipLaunchKernelGGL((conv3x3_direct_kernel<CO, false>), grid, dim3(kThreads), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -737,11 +740,16 @@ WgradPlan plan_wgrad(int n, int cin, int h, int w, int cout) {
 template <int BM, int BN>
 void launch_mfma(const ConvArgs &a, const ConvPlan &p, hipStream_t s) {
     const dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)p.co_tiles, (unsigned)(a.n * p.slices));
-    // float4 weight loads need 16-byte aligned rows of Cout floats
-    if (a.cout % 4 == 0 && ((uintptr_t)a.w & 15u) == 0)
-        hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, true>), grid, dim3(kThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, false>), grid, dim3(kThreads), 0, s, a);
+    // float4 weight loads need 16-byte aligned rows of Cout floats; the input
+    // gate is its own instantiation (no null test between the loads)
+    const bool vec = a.cout % 4 == 0 && ((uintptr_t)a.w & 15u) == 0;
+    if (a.gate) {
+        if (vec) hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, true, true>), grid, dim3(kThreads), 0, s, a);
+        else hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, false, true>), grid, dim3(kThreads), 0, s, a);
+    } else {
+        if (vec) hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, true, false>), grid, dim3(kThreads), 0, s, a);
+        else hipLaunchKernelGGL((conv3x3_mfma_kernel<BM, BN, kKC, false, false>), grid, dim3(kThreads), 0, s, a);
+    }
 }
 
 }  // namespace

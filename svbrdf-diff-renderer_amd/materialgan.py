@@ -28,12 +28,12 @@ weights:
   latent_to_textures (clamps of src/materialgan.py:91-100) and optim.
 
 The 3x3 convolutions of both networks run on the matrix-core kernels of
-:mod:`svbrdf_amd.convnet` (forward and input gradient; the generator's
-weight gradient, which the latent's gradient needs, is torch's), with the
-noise, bias and leaky-ReLU epilogue fused; the x2 upsampling is a HIP kernel
-too.  The renderer is :class:`svbrdf_amd.Microfacet` (forward and adjoint
-kernels).  The 1x1 toRGB convolutions (9 output channels) and the
-fully connected mapping layers stay torch operations.
+:mod:`svbrdf_amd.convnet` (forward, input gradient and the generator's
+weight gradient, which the latent's gradient needs), with the noise, bias
+and leaky-ReLU epilogue fused; the x2 upsampling is a HIP kernel too.  The
+renderer is :class:`svbrdf_amd.Microfacet` (forward and adjoint kernels).
+The 1x1 toRGB convolutions (9 output channels: plain GEMMs, torch.matmul)
+and the fully connected mapping layers stay torch operations.
 """
 import datetime
 import math
@@ -92,7 +92,9 @@ class _ModConv(torch.nn.Module):
     def forward(self, x, w, bias=None, noise=None, noise_strength=0.0, act="none"):
         wt = self.modulated_weight(w)
         if self.k == 1:
-            return F.conv2d(x, wt)
+            # a 1x1 convolution is a plain GEMM: [Cout, Cin] x [N, Cin, H W]
+            n, c, h, wd = x.shape
+            return torch.matmul(wt.view(wt.shape[0], c), x.reshape(n, c, h * wd)).view(n, -1, h, wd)
         if self.upsample:
             x = convnet.upsample2(x)
         return convnet.conv3x3(x, wt, bias, noise, noise_strength, act)
