@@ -38,6 +38,7 @@ using svbrdf_internal::fail;
 namespace {
 
 constexpr int kBlock = 256;
+typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kTile = 16;                 // resize: 16 x 16 output pixels per block
 constexpr int kMaxSide = 16384;           // largest 8-bit image side accepted
 constexpr int kMaxRes = 7680;             // as the renderer's maximum resolution
@@ -368,8 +369,11 @@ __device__ __forceinline__ void store_px4(uint8_t *base, size_t group, const uin
         w[k] = (uint32_t)b[4 * k] | ((uint32_t)b[4 * k + 1] << 8) | ((uint32_t)b[4 * k + 2] << 16) |
                ((uint32_t)b[4 * k + 3] << 24);
 }
+// Non-temporal: the decoded planes stream past the caches to their consumer
+// (images_from_u8 22.4 -> 20.1 us, textures_from_u8 11.6 -> 9.4 us at 1024^2).
 __device__ __forceinline__ void store4(float *plane, size_t q, float a, float b, float c, float d) {
-    *reinterpret_cast<float4 *>(plane + q) = make_float4(a, b, c, d);
+    const v4f v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(plane + q));
 }
 __device__ __forceinline__ float4 load4(const float *plane, size_t q) {
     return *reinterpret_cast<const float4 *>(plane + q);
@@ -579,8 +583,11 @@ __global__ __launch_bounds__(kBlock) void upsample2x_rows_kernel(const float *__
             b[k] = 0.75f * h[r + 1][k] + 0.25f * h[r + 2][k];
         }
         const size_t o = (size_t)(2u * i) * (2u * res) + 4u * q;
-        *reinterpret_cast<float4 *>(d + o) = make_float4(a[0], a[1], a[2], a[3]);
-        *reinterpret_cast<float4 *>(d + o + 2u * res) = make_float4(b[0], b[1], b[2], b[3]);
+        // non-temporal: the output streams past the caches (8.2-8.5 against
+        // 9.4-10.1 us at 512^2 -> 1024^2 x 9)
+        const v4f va = {a[0], a[1], a[2], a[3]}, vb = {b[0], b[1], b[2], b[3]};
+        __builtin_nontemporal_store(va, reinterpret_cast<v4f *>(d + o));
+        __builtin_nontemporal_store(vb, reinterpret_cast<v4f *>(d + o + 2u * res));
     }
 }
 
