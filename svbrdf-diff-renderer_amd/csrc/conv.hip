@@ -28,9 +28,10 @@
 //                         pixel slices.
 //   conv3x3_direct_kernel the same convolution for at most 8 output channels
 //                         (the input gradient of a layer that reads the RGB
-//                         image): two pixels per thread, all output channels
-//                         in registers, on the vector ALUs -- an MFMA tile of
-//                         64 output channels would be mostly padding.
+//                         image): input tiles staged in LDS, two pixels per
+//                         thread, all output channels in registers, on the
+//                         vector ALUs -- an MFMA tile of 64 output channels
+//                         would be mostly padding.
 //   upsample2_blur_kernel x2 upsampling by zero insertion and the [1, 3, 3, 1]
 //                         binomial filter (gain 4): out(2p) = (x(p-1) + 3 x(p))/4,
 //                         out(2p+1) = (3 x(p) + x(p+1))/4 per axis, zero outside
@@ -328,56 +329,74 @@ __global__ __launch_bounds__(kThreads) void conv_split_sum_kernel(ConvArgs a) {
 }
 
 // Direct convolution for Cout = CO <= 8: a 64 x 8 pixel block, two
-// vertically adjacent pixels per thread (12 input loads per channel for 18
-// taps), the 2 x CO sums in registers.  The weights of one input channel and
-// tap are the same for every thread: scalar loads.  Out-of-image taps load a
-// clamped in-image address and select 0, so the loop has no branches.
+// vertically adjacent pixels per thread, the 2 x CO sums in registers.  Per
+// chunk of 8 input channels the block stages the 66 x 10 input tiles (with
+// the one-pixel halo, zero outside the image; times the gate, when there is
+// one) in LDS with coalesced loads, and every thread reads its 4 x 3 window
+// per channel from there.  The weights of one input channel and tap are the
+// same for every thread: scalar loads.
 template <int CO, bool GATE>
 __global__ __launch_bounds__(kThreads) void conv3x3_direct_kernel(ConvArgs a) {
+    constexpr int TX = 64, TY = 8, HX = TX + 2, HY = TY + 2, CC = 8;
+    __shared__ float tile[CC][HY * HX];
     const int H = a.h, W = a.w_;
-    const int xx = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y0 = blockIdx.y * 8 + (threadIdx.x >> 6) * 2;
+    const int bx0 = blockIdx.x * TX, by0 = blockIdx.y * TY;
+    const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) * 2;   // column, first of the two rows
     const int img = blockIdx.z;
-    if (xx >= W || y0 >= H) return;
     const size_t HW = (size_t)H * W;
-    int off[4][3];
-    bool in[4][3];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int y2 = y0 + r - 1, x2 = xx + c - 1;
-            in[r][c] = y2 >= 0 && y2 < H && x2 >= 0 && x2 < W;
-            off[r][c] = min(max(y2, 0), H - 1) * W + min(max(x2, 0), W - 1);
-        }
+    const float *xi = a.x + (size_t)img * a.cin * HW;
+    const float *gi = GATE ? a.gate + (size_t)img * a.cin * HW : nullptr;
+    const size_t tap_stride = (size_t)a.cin * CO;
     float acc0[CO], acc1[CO];
 #pragma unroll
     for (int co = 0; co < CO; ++co) acc0[co] = acc1[co] = 0.0f;
-    const float *xc = a.x + (size_t)img * a.cin * HW;
-    const float *gc = GATE ? a.gate + (size_t)img * a.cin * HW : nullptr;
-    const size_t tap_stride = (size_t)a.cin * CO;
-#pragma unroll 2
-    for (int ci = 0; ci < a.cin; ++ci, xc += HW, gc += GATE ? HW : 0) {
-        float v[4][3];
+    constexpr int NE = CC * HY * HX, PER = (NE + kThreads - 1) / kThreads;
+    for (int c0 = 0; c0 < a.cin; c0 += CC) {
+        // every load of the chunk first (branch-free: a clamped address and a
+        // select), then the LDS stores
+        float xv[PER], gv[PER];
+        bool ok[PER];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < PER; ++k) {
+            const int e = threadIdx.x + k * kThreads;
+            const int cc = e / (HY * HX), rem = e - cc * (HY * HX);
+            const int r = rem / HX, c = rem - r * HX;
+            const int ci = c0 + cc, yy = by0 - 1 + r, xx = bx0 - 1 + c;
+            ok[k] = e < NE && ci < a.cin && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            const size_t o = ok[k] ? (size_t)ci * HW + (size_t)yy * W + xx : 0;
+            xv[k] = xi[o];
+            if constexpr (GATE) gv[k] = gi[o];
+        }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float raw = xc[off[r][c]];
-                if constexpr (GATE) raw = raw * (gc[off[r][c]] > 0.0f ? a.gate_pos : a.gate_neg);
-                v[r][c] = in[r][c] ? raw : 0.0f;
-            }
-        const float *wc = a.w + (size_t)ci * CO;
+        for (int k = 0; k < PER; ++k) {
+            const int e = threadIdx.x + k * kThreads;
+            float v = xv[k];
+            if constexpr (GATE) v = v * (gv[k] > 0.0f ? a.gate_pos : a.gate_neg);
+            if (e < NE) (&tile[0][0])[e] = ok[k] ? v : 0.0f;
+        }
+        __syncthreads();
+        const int nc = min(CC, a.cin - c0);
+        for (int cc = 0; cc < nc; ++cc) {
+            float v[4][3];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
+            for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int co = 0; co < CO; ++co) {
-                const float wv = wc[t * tap_stride + co];
-                acc0[co] += v[t / 3][t % 3] * wv;
-                acc1[co] += v[t / 3 + 1][t % 3] * wv;
+                for (int c = 0; c < 3; ++c) v[r][c] = tile[cc][(ty + r) * HX + tx + c];
+            const float *wc = a.w + (size_t)(c0 + cc) * CO;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+#pragma unroll
+                for (int co = 0; co < CO; ++co) {
+                    const float wv = wc[t * tap_stride + co];
+                    acc0[co] += v[t / 3][t % 3] * wv;
+                    acc1[co] += v[t / 3 + 1][t % 3] * wv;
+                }
             }
         }
+        __syncthreads();
     }
+    const int xx = bx0 + tx, y0 = by0 + ty;
+    if (xx >= W || y0 >= H) return;
     float *yi = a.y + (size_t)img * CO * HW;
     const size_t p0 = (size_t)y0 * W + xx;
 #pragma unroll
