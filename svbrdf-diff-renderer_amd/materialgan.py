@@ -223,6 +223,7 @@ class MaterialGANOptim:
         self.loss_small_feature = VGGLoss(self.device, np.array([8, 8, 2, 1]) / 19)
         self.noises = None
         self.latent = None
+        self._graphs = {}
 
     def init_from(self, ckp=(), seed=0):
         """Latent W+ from a random z through mapping + truncation (or a saved
@@ -240,8 +241,10 @@ class MaterialGANOptim:
                            for n in torch.load(ckp[1], map_location=self.device, weights_only=True)]
         else:
             self.noises = [n.requires_grad_(True) for n in self.net_obj.make_noises(gen)]
+        self._graphs = {}   # recorded for the previous latent / noise tensors
 
     def load_targets(self, targets):
+        self._graphs = {}
         self.targets = targets
         self.loss_large_feature.load(targets)
         self.loss_small_feature.load(targets)
@@ -266,7 +269,73 @@ class MaterialGANOptim:
 
     save_loss = staticmethod(SvbrdfOptim.save_loss)
 
-    def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False, save_every=200):
+    # ------------------------------------------------------------ one epoch
+    @staticmethod
+    def _adam_first_step(params, exp_avg, exp_avg_sq, lr):
+        """The step a fresh torch.optim.Adam(params, lr, betas=(0.9, 0.999))
+        takes: its foreach implementation on the GPU (torch/optim/adam.py,
+        _multi_tensor_adam, step 1, capturable=False) op for op, with the same
+        Python scalars, so the same bits -- written out so that the epoch can
+        be recorded in a HIP graph."""
+        grads = [p.grad for p in params]
+        torch._foreach_zero_(exp_avg)
+        torch._foreach_zero_(exp_avg_sq)
+        torch._foreach_lerp_(exp_avg, grads, 1 - 0.9)
+        torch._foreach_mul_(exp_avg_sq, 0.999)
+        torch._foreach_addcmul_(exp_avg_sq, grads, grads, 1 - 0.999)
+        bc1 = 1 - 0.9 ** 1.0
+        bc2 = 1 - 0.999 ** 1.0
+        denom = torch._foreach_sqrt(exp_avg_sq)
+        torch._foreach_div_(denom, [bc2 ** 0.5] * len(params))
+        torch._foreach_add_(denom, 1e-8)
+        torch._foreach_addcdiv_(params, exp_avg, denom, [(lr / bc1) * -1] * len(params))
+
+    def _epoch(self, which, lr):
+        """One epoch of optim() (a fresh Adam on the latent ("L") or the noise
+        maps ("N")); returns the image and feature losses as device scalars."""
+        params = [self.latent] if which == "L" else list(self.noises)
+        for q in params:
+            q.grad = None                                            # optimizer.zero_grad()
+        textures = self.latent_to_textures(self.latent)
+        rendereds = self.renderer_obj.eval(textures)
+        loss_image = self.compute_image_loss(rendereds)
+        loss_feature = self.compute_feature_loss(rendereds, which) * 0.1
+        (loss_image + loss_feature).backward()
+        key = ("moments", which)
+        if key not in self._graphs:
+            self._graphs[key] = ([torch.zeros_like(q) for q in params], [torch.zeros_like(q) for q in params])
+        with torch.no_grad():
+            self._adam_first_step(params, *self._graphs[key], lr)
+        return loss_image.detach(), loss_feature.detach()
+
+    def _graph_epoch(self, which, lr):
+        """_epoch replayed from a HIP graph: the first epoch of each kind (and
+        learning rate) runs on a side stream as the warm-up the capture needs,
+        then the epoch is recorded; later epochs of that kind are one graph
+        launch.  The iteration launches ~1,500 kernels, most of them small,
+        so this removes the host's launch time from every epoch."""
+        key = (which, float(lr))
+        if key in self._graphs:
+            g, li, lf = self._graphs[key]
+            g.replay()
+            return li.clone(), lf.clone()
+        dev = self.device
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            li, lf = self._epoch(which, lr)
+        torch.cuda.current_stream(dev).wait_stream(side)
+        # the recorded backward allocates every gradient from the graph's pool
+        # and rewrites it on each replay
+        for q in [self.latent] + list(self.noises):
+            q.grad = None
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            gli, glf = self._epoch(which, lr)
+        self._graphs[key] = (g, gli, glf)
+        return li, lf
+
+    def optim(self, epochs, lr, svbrdf_obj=None, optim_light=False, save_every=200, graph=True):
         """epochs = (total, l_epochs, n_epochs): l_epochs optimising the latent
         (feature loss "L") then n_epochs the noise ("N"), repeated; a fresh
         Adam each epoch as the reference builds it (src/materialgan.py:
@@ -275,7 +344,9 @@ class MaterialGANOptim:
         every 200th and the last under optimize_dir/tmp/<timestamp>/<epoch>/
         (:103-105, 157-169).  Afterwards ``textures`` holds the maps of the
         final latent and noise, and ``loss`` / ``loss_image`` the last epoch's
-        losses (:171-173).  Returns the per-epoch (image, feature) losses."""
+        losses (:171-173).  Returns the per-epoch (image, feature) losses.
+        graph=True (without optim_light) replays each epoch from a HIP graph
+        per kind: the same kernels and Adam step, bit-identical results."""
         total, l_epochs, n_epochs = epochs
         cycle = l_epochs + n_epochs
         tmp_dir = None
@@ -290,21 +361,26 @@ class MaterialGANOptim:
             if svbrdf_obj is not None:
                 svbrdf_obj.cl[2] = light
         image_losses, feature_losses = [], []
+        use_graph = graph and light is None
         for epoch in range(total):
             which = "L" if (epoch % cycle) < l_epochs else "N"
-            params = [self.latent] if which == "L" else list(self.noises)
-            if light is not None:
-                params = params + [light]
-                self.renderer_obj.update_light(light)
-            optimizer = torch.optim.Adam(params, lr=lr, betas=(0.9, 0.999))
-            textures = self.latent_to_textures(self.latent)
-            rendereds = self.renderer_obj.eval(textures)
-            loss_image = self.compute_image_loss(rendereds)
-            loss_feature = self.compute_feature_loss(rendereds, which) * 0.1
-            loss = loss_image + loss_feature
-            optimizer.zero_grad()
-            loss.backward()
-            optimizer.step()
+            if use_graph:
+                loss_image, loss_feature = self._graph_epoch(which, lr)
+                loss = loss_image + loss_feature
+            else:
+                params = [self.latent] if which == "L" else list(self.noises)
+                if light is not None:
+                    params = params + [light]
+                    self.renderer_obj.update_light(light)
+                optimizer = torch.optim.Adam(params, lr=lr, betas=(0.9, 0.999))
+                textures = self.latent_to_textures(self.latent)
+                rendereds = self.renderer_obj.eval(textures)
+                loss_image = self.compute_image_loss(rendereds)
+                loss_feature = self.compute_feature_loss(rendereds, which) * 0.1
+                loss = loss_image + loss_feature
+                optimizer.zero_grad()
+                loss.backward()
+                optimizer.step()
             image_losses.append(loss_image.detach())
             feature_losses.append(loss_feature.detach())
             if tmp_dir is not None and ((epoch + 1) % save_every == 0 or epoch == 0 or epoch == total - 1):

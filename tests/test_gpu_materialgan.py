@@ -96,3 +96,35 @@ def test_materialgan_loop_runs_and_alternates(pkg):
     assert torch.equal(opt.latent.detach(), lat1)
     assert any(not torch.equal(n.detach(), m) for n, m in zip(opt.noises, noise0))
     assert opt.textures.shape == (1, 9, res, res)
+
+
+def test_graph_epochs_match_eager_adam(pkg):
+    """optim(graph=True) replays each epoch from a HIP graph per kind (the
+    Adam step written out as the foreach ops torch.optim.Adam runs); from the
+    same start it must give the eager loop's latent, noise maps and losses
+    bit for bit, across latent -> noise -> latent switches and replays of both
+    graphs, and twice in a row (a second optim() call reuses the graphs)."""
+    res = 256
+    cam, light, P, size = scene(pkg, res)
+    cl = [torch.tensor(cam), torch.tensor(light), torch.tensor(P)]
+    r = pkg.Microfacet(res, 9, size, cl, DEV)
+    from svbrdf_amd import materialgan as MG
+    opt_t = MG.MaterialGANOptim(DEV, r, seed=7)
+    opt_t.init_from(seed=123)
+    with torch.no_grad():
+        targets = r.eval(opt_t.latent_to_textures(opt_t.latent))
+    runs = []
+    for graph in (False, True):
+        opt = MG.MaterialGANOptim(DEV, r, seed=7)
+        opt.init_from(seed=5)
+        opt.load_targets(targets)
+        li, lf = opt.optim((7, 2, 2), 0.02, graph=graph)
+        li2, lf2 = opt.optim((3, 1, 1), 0.01, graph=graph)
+        torch.cuda.synchronize()
+        runs.append((opt.latent.detach().clone(), [n.detach().clone() for n in opt.noises], li + li2, lf + lf2,
+                     opt.textures.clone()))
+    (la, na, lia, lfa, ta), (lb, nb, lib, lfb, tb) = runs
+    assert torch.equal(la, lb)
+    assert all(torch.equal(x, y) for x, y in zip(na, nb))
+    assert lia == lib and lfa == lfb
+    assert torch.equal(ta, tb)
