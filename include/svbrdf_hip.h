@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 8
+#define SVBRDF_ABI_VERSION 9
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -416,6 +416,14 @@ int svbrdf_conv3x3_input_grad(const float *grad_y, const float *y, int activatio
 size_t svbrdf_conv3x3_wgrad_workspace_bytes(int n, int cin, int height, int width, int cout);
 int svbrdf_conv3x3_wgrad(const float *x, const float *grad_z, int n, int cin, int height, int width, int cout,
                          float *grad_w, void *workspace, size_t workspace_bytes, void *stream);
+
+/* 2 x 2 average pooling with stride 2 of `planes` planes of height x width
+ * (y [planes, height / 2, width / 2], a trailing odd row / column dropped):
+ * the AvgPool2d(kernel_size=2) the VGG feature loss uses in place of max
+ * pooling (src/descriptor.py:15-18), bit-identical to torch's (window summed
+ * row by row, then / 4); and its adjoint (grad_x [planes, height, width]). */
+int svbrdf_avgpool2(const float *x, int planes, int height, int width, float *y, void *stream);
+int svbrdf_avgpool2_adjoint(const float *grad_y, int planes, int height, int width, float *grad_x, void *stream);
 
 /* x2 upsampling of `planes` planes of height x width: zero insertion and the
  * [1, 3, 3, 1] binomial filter with gain 4 (UpsamplingLayer,

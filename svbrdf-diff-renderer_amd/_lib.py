@@ -15,7 +15,7 @@ _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("SVBRDF_AMD_LIB") or _PKG / "libsvbrdf_hip.so")
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 OK = 0
 EINVAL = -1
@@ -91,6 +91,8 @@ SIGNATURES = {
     "svbrdf_conv3x3_wgrad_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 5),
     "svbrdf_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, _vp, _vp, ctypes.c_size_t, _vp]),
+    "svbrdf_avgpool2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "svbrdf_avgpool2_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "svbrdf_upsample2_blur": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "svbrdf_upsample2_blur_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
 }

@@ -46,7 +46,7 @@ class VGGShapedFeatures(torch.nn.Module):
         feats = []
         for step in self.plan:
             if step == "P":
-                x = torch.nn.functional.avg_pool2d(x, 2)
+                x = convnet.avgpool2(x)
             else:
                 x = self.convs[step](x)
                 if step in STYLE_LAYERS:

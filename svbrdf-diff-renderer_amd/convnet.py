@@ -186,6 +186,34 @@ def upsample2(x):
     return Upsample2Function.apply(x)
 
 
+class AvgPool2Function(torch.autograd.Function):
+    """F.avg_pool2d(x, 2) (src/descriptor.py:15-18 puts AvgPool2d(2) in
+    place of VGG's max pooling) on the HIP kernels: same bits forward, and
+    the adjoint writes dL/dy / 4 to each window."""
+
+    @staticmethod
+    def forward(ctx, x):
+        x = x.contiguous()
+        _check(x, "x")
+        n, c, h, w = (int(s) for s in x.shape)
+        y = torch.empty((n, c, h // 2, w // 2), dtype=torch.float32, device=x.device)
+        _lib.call("svbrdf_avgpool2", _lib.ptr(x), n * c, h, w, _lib.ptr(y), _lib.stream_handle(x.device))
+        ctx.shape = (n, c, h, w)
+        return y
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        g = grad_y.contiguous()
+        n, c, h, w = ctx.shape
+        gx = torch.empty((n, c, h, w), dtype=torch.float32, device=g.device)
+        _lib.call("svbrdf_avgpool2_adjoint", _lib.ptr(g), n * c, h, w, _lib.ptr(gx), _lib.stream_handle(g.device))
+        return gx
+
+
+def avgpool2(x):
+    return AvgPool2Function.apply(x)
+
+
 class _FrozenConvFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, layer):

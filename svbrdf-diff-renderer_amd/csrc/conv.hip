@@ -564,6 +564,46 @@ __global__ __launch_bounds__(kThreads) void wgrad_sum_kernel(const float *__rest
 }
 
 // ---------------------------------------------------------------------------
+// 2 x 2 average pooling, stride 2 (torch.nn.AvgPool2d(kernel_size=2), which
+// the VGG feature loss puts in place of max pooling), and its adjoint.  One
+// thread per output pixel; the window is summed row by row, left to right,
+// then divided by 4, as torch's kernel does, so the outputs are the same bits.
+// A trailing odd row or column is dropped (its gradient is zero).
+__global__ __launch_bounds__(kThreads) void avgpool2_kernel(const float *__restrict__ x, int planes, int h, int w,
+                                                            float *__restrict__ y) {
+    const int ho = h / 2, wo = w / 2;
+    const size_t total = (size_t)planes * ho * wo;
+    const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (e >= total) return;
+    const int j = (int)(e % wo), i = (int)((e / wo) % ho);
+    const size_t pl = e / ((size_t)ho * wo);
+    const float *r0 = x + (pl * h + 2 * i) * (size_t)w + 2 * j;
+    const float *r1 = r0 + w;
+    float v = r0[0];
+    v += r0[1];
+    v += r1[0];
+    v += r1[1];
+    y[e] = v / 4.0f;
+}
+
+__global__ __launch_bounds__(kThreads) void avgpool2_adjoint_kernel(const float *__restrict__ gy, int planes, int h,
+                                                                    int w, float *__restrict__ gx) {
+    // one thread per input pair (2 columns of one row): g / 4 where the pair
+    // lies in a window, 0 in a dropped row or column
+    const int ho = h / 2, wo = w / 2, wp = (w + 1) / 2;
+    const size_t total = (size_t)planes * h * wp;
+    const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (e >= total) return;
+    const int jp = (int)(e % wp), row = (int)((e / wp) % h);
+    const size_t pl = e / ((size_t)h * wp);
+    const int i = row / 2;
+    const float g = (i < ho && jp < wo) ? gy[(pl * ho + i) * (size_t)wo + jp] / 4.0f : 0.0f;
+    float *o = gx + (pl * h + row) * (size_t)w + 2 * jp;
+    o[0] = g;
+    if (2 * jp + 1 < w) o[1] = g;
+}
+
+// ---------------------------------------------------------------------------
 // x2 upsampling with the [1, 3, 3, 1] filter and its adjoint.  Zero insertion
 // (x(p) at 2p), padding 2 before and 1 after, and the filter [1, 3, 3, 1]/4
 // per axis give, per axis,
@@ -895,6 +935,30 @@ int svbrdf_conv3x3_wgrad(const float *x, const float *grad_z, int n, int cin, in
         hipLaunchKernelGGL(wgrad_sum_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0,
                            s, (const float *)workspace, p.slices, total, grad_w);
     }
+    return check_launch(what);
+}
+
+int svbrdf_avgpool2(const float *x, int planes, int height, int width, float *y, void *stream) {
+    const char *what = "svbrdf_avgpool2";
+    if (!x || !y) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (planes < 1 || height < 2 || width < 2 || height > kMaxSide || width > kMaxSide ||
+        (size_t)planes * height * width > ((size_t)1 << 34))
+        return fail(SVBRDF_ESHAPE, "%s: planes=%d height=%d width=%d", what, planes, height, width);
+    const size_t total = (size_t)planes * (height / 2) * (width / 2);
+    hipLaunchKernelGGL(avgpool2_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       (hipStream_t)stream, x, planes, height, width, y);
+    return check_launch(what);
+}
+
+int svbrdf_avgpool2_adjoint(const float *grad_y, int planes, int height, int width, float *grad_x, void *stream) {
+    const char *what = "svbrdf_avgpool2_adjoint";
+    if (!grad_y || !grad_x) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (planes < 1 || height < 2 || width < 2 || height > kMaxSide || width > kMaxSide ||
+        (size_t)planes * height * width > ((size_t)1 << 34))
+        return fail(SVBRDF_ESHAPE, "%s: planes=%d height=%d width=%d", what, planes, height, width);
+    const size_t total = (size_t)planes * height * ((width + 1) / 2);
+    hipLaunchKernelGGL(avgpool2_adjoint_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, (hipStream_t)stream, grad_y, planes, height, width, grad_x);
     return check_launch(what);
 }
 

@@ -197,7 +197,7 @@ class VGGLoss(torch.nn.Module):
         feats = []
         for step in self.plan:
             if step == "P":
-                x = F.avg_pool2d(x, 2)
+                x = convnet.avgpool2(x)
                 continue
             x = self.convs[step](x)
             if step in self.HOOKS:

@@ -206,6 +206,22 @@ def test_upsample2_and_adjoint(pkg):
     assert torch.allclose(up[:, :, 1:-1, 1:-1], torch.ones_like(up[:, :, 1:-1, 1:-1]))
 
 
+@pytest.mark.parametrize("shape", [(9, 64, 256, 256), (2, 3, 7, 10), (1, 5, 9, 9), (1, 1, 2, 2)])
+def test_avgpool2_matches_torch_bit_for_bit(pkg, shape):
+    """svbrdf_avgpool2 and its adjoint against F.avg_pool2d(x, 2) and its
+    autograd: the same bits (window summed row by row, then / 4), odd sizes
+    dropping their last row / column."""
+    from svbrdf_amd import convnet
+    x = torch.randn(shape, generator=torch.Generator().manual_seed(sum(shape))).to(DEV)
+    xa, xb = x.clone().requires_grad_(True), x.clone().requires_grad_(True)
+    ya, yb = convnet.avgpool2(xa), F.avg_pool2d(xb, 2)
+    assert torch.equal(ya, yb)
+    gy = torch.randn(ya.shape, generator=torch.Generator().manual_seed(3)).to(DEV)
+    ya.backward(gy)
+    yb.backward(gy)
+    assert torch.equal(xa.grad, xb.grad)
+
+
 def _masks_of(layers):
     """Forward hooks recording each layer's activation branch (output > 0,
     the sign of the pre-activation); returns (masks, remove)."""
