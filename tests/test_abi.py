@@ -183,3 +183,25 @@ def test_conv_workspace_plan(lib):
     assert lib.svbrdf_conv3x3(f, 1, 512, 4, 4, f, 512, None, None, 0.0, 0, f, f, need - 4, None) == -3
     assert lib.svbrdf_conv3x3(f, 1, 0, 4, 4, f, 512, None, None, 0.0, 0, f, None, 0, None) == -2
     assert lib.svbrdf_conv3x3(f, 1, 8, 4, 4, f, 8, None, None, 0.0, 7, f, None, 0, None) == -1
+
+
+def test_conv_weight_gradient_plan_and_validation(lib):
+    """svbrdf_conv3x3_wgrad splits the pixel sum only where the channel tiles
+    are few and the pixels many: 128 -> 128 channels at 256^2 runs in slices
+    whose partial sums are [slices][128][128][9]; 512 -> 512 at 4 x 4 (one
+    16-pixel chunk) needs no workspace.  Bad shapes and a short workspace are
+    refused before any launch; so are bad pooling and gated-gradient calls."""
+    per = 128 * 128 * 9 * 4
+    need = lib.svbrdf_conv3x3_wgrad_workspace_bytes(1, 128, 256, 256, 128)
+    assert need % per == 0 and 2 <= need // per <= 256
+    assert lib.svbrdf_conv3x3_wgrad_workspace_bytes(1, 512, 4, 4, 512) == 0
+    assert lib.svbrdf_conv3x3_wgrad_workspace_bytes(1, 0, 4, 4, 512) == 0
+    f = _fake()
+    assert lib.svbrdf_conv3x3_wgrad(f, f, 1, 128, 256, 256, 128, f, None, 0, None) == -3
+    assert lib.svbrdf_conv3x3_wgrad(f, f, 1, 128, 256, 256, 128, f, f, need - 4, None) == -3
+    assert lib.svbrdf_conv3x3_wgrad(f, f, 0, 8, 4, 4, 8, f, None, 0, None) == -2
+    assert lib.svbrdf_conv3x3_wgrad(None, f, 1, 8, 4, 4, 8, f, None, 0, None) == -1
+    assert lib.svbrdf_avgpool2(f, 1, 1, 4, f, None) == -2
+    assert lib.svbrdf_avgpool2_adjoint(None, 1, 4, 4, f, None) == -1
+    assert lib.svbrdf_conv3x3_input_grad(f, f, 9, 1, 8, 4, 4, f, 8, f, None, 0, None) == -1   # activation
+    assert lib.svbrdf_conv3x3_input_grad(f, None, 1, 1, 8, 4, 4, f, 8, f, None, 0, None) == -1
