@@ -300,7 +300,7 @@ __device__ __forceinline__ void shade_backward(const Maps &m, const Geom &g, con
 // pixel constant Maps::Fc, and the three dot-product gradients act along v.
 struct ShadeC {
     float v[3];
-    float nv_raw, nv, c2, den, iqd, D, idv, g1, G, iq, Dq1, DGq, inv_d2, nl_d2;
+    float nv_raw, nv, c2, den, iqd, D, idv, g1, iq, Dq1, DGq, inv_d2, nl_d2;
 };
 
 __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, float cx, float cy, float cz,
@@ -317,19 +317,20 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     s.nv = tmax0(s.nv_raw);
     // GGX with 1 - (n.h)^2 = |v - (n.v) n|^2 for unit n, v: the components
     // are O(sin) with O(ulp) absolute error, so there is no cancellation at
-    // the highlight (1 - c^2 itself would lose all digits as c -> 1)
+    // the highlight (1 - c^2 itself would lose all digits as c -> 1).  For
+    // n.v < 0 the clamped c = 0 makes 1 - c^2 = 1, but every use of den is
+    // then multiplied by c^2 = 0 (D G / q through g1 = 0, the gradient
+    // through the masked dL/dimg = 0), so |w|^2 < 1 there changes nothing.
     s.c2 = s.nv * s.nv;
     const float wx = s.v[0] - s.nv_raw * m.n[0];
     const float wy = s.v[1] - s.nv_raw * m.n[1];
     const float wz = s.v[2] - s.nv_raw * m.n[2];
-    const float one_m_c2 = s.nv_raw < 0.0f ? 1.0f : wx * wx + wy * wy + wz * wz;
-    s.den = s.c2 * m.a2 + one_m_c2;
+    s.den = s.c2 * m.a2 + (wx * wx + wy * wy + wz * wz);
     s.iqd = frcp(kPi * (s.den * s.den) + kEps);
     s.D = m.a2 * s.iqd;
     // Smith: G = G1(n.v)^2
     s.idv = frcp(s.nv * m.omk + m.k);
     s.g1 = s.nv * s.idv;
-    s.G = s.g1 * s.g1;
     s.iq = frcp(4.0f * s.nv * s.nv + kEps);
     // D G / q as (D g1 / q) g1: the first factor is the adjoint's D G /(q n.v)
     s.Dq1 = s.D * s.iq * s.g1;
@@ -353,9 +354,9 @@ __device__ __forceinline__ void shade_coloc_fwd(const Maps &m, float px, float p
     const float nv_raw = m.n[0] * v0 + m.n[1] * v1 + m.n[2] * v2;
     const float nv = tmax0(nv_raw);
     const float c2 = nv * nv;
+    // |w|^2 = 1 - c^2 also for n.v < 0, where c2 = 0 makes DGq = 0 (shade_coloc)
     const float wx = v0 - nv_raw * m.n[0], wy = v1 - nv_raw * m.n[1], wz = v2 - nv_raw * m.n[2];
-    const float one_m_c2 = nv_raw < 0.0f ? 1.0f : wx * wx + wy * wy + wz * wz;
-    const float den = c2 * m.a2 + one_m_c2;
+    const float den = c2 * m.a2 + (wx * wx + wy * wy + wz * wz);
     const float sd = nv * m.omk + m.k;
     const float q = (kPi * (den * den) + kEps) * (sd * sd) * (4.0f * c2 + kEps);
     DGq = (m.a2 * c2) * frcp(q);
@@ -376,23 +377,30 @@ __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, c
         a.T[c] += gf[c] * s.DGq;
         W = c == 0 ? gf[c] * Fw[c] : W + gf[c] * Fw[c];
     }
-    // f2 = D G F / q with D, G, q functions of n.v (= n.l = n.h) and W =
-    // sum_c dL/df_c F_c; WD = W D G / q.  With den and iqd of GGX and idv of
-    // Smith:
-    //   d ln D / d nv = -4 pi (a2 - 1) den iqd nv    (dden/dnv = 2 (a2 - 1) nv)
+    // f2 = D G F / q with D, G, q functions of n.v (= n.l = n.h), W =
+    // sum_c dL/df_c F_c and WD = W D G / q.  With den and iqd of GGX and idv
+    // of Smith:
+    //   d ln D / d nv = 4 pi (1 - a2) den iqd nv     (dden/dnv = 2 (a2 - 1) nv)
     //   d ln G / d nv = 2 k idv / nv                 (G = g1^2, g1 = nv idv)
     //   d ln q / d nv = 8 nv iq
-    //   dL/dk        = -2 WD (1 - nv) idv            (d ln g1/dk = -(1 - nv) idv)
-    //   dL/da2       = W G iq iqd (1 - 2 pi a2 den c2 iqd)
-    // The G term WD 2 k idv / nv is 2 k idv^2 W (D g1 / q): no 1/n.v.
-    const float WD = W * s.DGq;
+    // and WD nv = W Dq1 c2 idv, WD / nv = W Dq1 idv (Dq1 = D g1 / q), so
+    //   dL/dnv = W Dq1 idv (c2 (4 pi (1 - a2) den iqd - 8 iq) + 2 k idv)
+    // with no 1/n.v.  Roughness: alpha feeds a2 = alpha^2 and k = alpha/2 + eps,
+    //   d ln D / d a2 = 1/a2 - 2 pi den iqd c2       (dden/da2 = c2)
+    //   d ln G / d k  = -2 (1 - nv) idv
+    //   dL/dalpha = WD (2/alpha - 4 pi alpha den iqd c2 - (1 - nv) idv),
+    // 2/alpha for 2 alpha/a2, taken as 0 at alpha = 0 (there a2 = 0, D = 0
+    // and WD = 0; for alpha > 0 it is at most 2^111).  n.v < 0 needs no
+    // clamp mask: dL/dimg is masked to 0 for every channel there (img = 0 <
+    // eps), so W = 0, gnl = 0 and dL/dnv = 0 with every factor finite.
     const float di = s.den * s.iqd;
-    const float WDq1 = W * s.Dq1;
-    float gnv = gnl + WD * (s.nv * ((-4.0f * kPi) * (m.a2 - 1.0f) * di - 8.0f * s.iq)) +
-                (2.0f * m.k) * (s.idv * s.idv) * WDq1;
-    const float ga2 = W * s.iq * s.G * s.iqd * (1.0f - (2.0f * kPi) * m.a2 * di * s.c2);
-    a.A += ga2 * 2.0f * m.alpha - WD * (1.0f - s.nv) * s.idv;
-    gnv = s.nv_raw >= 0.0f ? gnv : 0.0f;
+    const float WDq1i = W * s.Dq1 * s.idv;
+    const float WD = W * s.DGq;
+    const float bn = s.c2 * ((4.0f * kPi) * (1.0f - m.a2) * di - 8.0f * s.iq) + (2.0f * m.k) * s.idv;
+    const float gnv = gnl + WDq1i * bn;
+    const float two_ia = m.alpha > 0.0f ? 2.0f * frcp(m.alpha) : 0.0f;
+    const float ba = two_ia - (4.0f * kPi) * m.alpha * (di * s.c2) - (1.0f - s.nv) * s.idv;
+    a.A += WD * ba;
 #pragma unroll
     for (int c = 0; c < 3; ++c) a.gn[c] += gnv * s.v[c];
 }
