@@ -78,15 +78,19 @@ struct Maps {
 constexpr float kSphgColoc = 0x1.60a77cp-13f;
 __device__ __forceinline__ float sphg_coloc() { return kSphgColoc; }
 
+// u = (t+1)/2 as one fma: the halving is exact, so fma(t, 1/2, 1/2) rounds
+// t/2 + 1/2 once, to the value of the reference's (t + 1)/2.
+__device__ __forceinline__ float map_unit(float t) { return fmaf(t, 0.5f, 0.5f); }
+
 // u^2.2 of u = (t+1)/2; u in {0} or >= 2^-25, so log2 is finite or -inf.
 __device__ __forceinline__ float map_pow(float t) {
-    const float u = (t + 1.0f) * 0.5f;
+    const float u = map_unit(t);
     return fexp2(kGamma * flog2(u));
 }
 
 // d(u^2.2)/dt = 2.2 u^1.2 / 2 (torch pow_backward: exponent * x^(exponent-1)).
 __device__ __forceinline__ float map_pow_grad(float t) {
-    const float u = (t + 1.0f) * 0.5f;
+    const float u = map_unit(t);
     return 0.5f * kGamma * fexp2(kGammaM1 * flog2(u));
 }
 
@@ -331,7 +335,7 @@ __device__ __forceinline__ void shade_coloc(const Maps &m, float px, float py, f
     // Smith: G = G1(n.v)^2
     s.idv = frcp(s.nv * m.omk + m.k);
     s.g1 = s.nv * s.idv;
-    s.iq = frcp(4.0f * s.nv * s.nv + kEps);
+    s.iq = frcp(4.0f * s.c2 + kEps);
     // D G / q as (D g1 / q) g1: the first factor is the adjoint's D G /(q n.v)
     s.Dq1 = s.D * s.iq * s.g1;
     s.DGq = s.Dq1 * s.g1;
@@ -368,6 +372,11 @@ __device__ __forceinline__ void shade_coloc_fwd(const Maps &m, float px, float p
 // scaled by dF/dspe = 1 - 2^(A+B) once per pixel (coloc_finish).  Fw are the
 // Fresnel factors paired with gf: F_c, or P_c F_c when the caller passes
 // gf / P_c (and scales a.S and a.T by P_c after the view loop).
+// coloc_gradients is its second half: the normal and roughness terms from
+// W = sum_c dL/df_c Fw_c and the direct dL/d(n.l), for callers that
+// accumulate a.S and a.T themselves.
+__device__ __forceinline__ void coloc_gradients(const Maps &m, const ShadeC &s, float W, float gnl, Accum &a);
+
 __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, const float gf[3], float gnl,
                                                const float Fw[3], Accum &a) {
     float W;
@@ -377,6 +386,10 @@ __device__ __forceinline__ void backward_coloc(const Maps &m, const ShadeC &s, c
         a.T[c] += gf[c] * s.DGq;
         W = c == 0 ? gf[c] * Fw[c] : W + gf[c] * Fw[c];
     }
+    coloc_gradients(m, s, W, gnl, a);
+}
+
+__device__ __forceinline__ void coloc_gradients(const Maps &m, const ShadeC &s, float W, float gnl, Accum &a) {
     // f2 = D G F / q with D, G, q functions of n.v (= n.l = n.h), W =
     // sum_c dL/df_c F_c and WD = W D G / q.  With den and iqd of GGX and idv
     // of Smith:

@@ -328,7 +328,11 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
         if constexpr (COLOC) {
             ShadeC s;
             shade_coloc(m, px.px, px.py, vp.cx, vp.cy, vp.cz, s);
-            float gf[3], gnl = 0.f;
+            // dL/d(P_c f_c) = gimg_c nl_d2 and dL/d(P_c F_c) = gimg_c nl_d2 DGq
+            // (a.S, a.T), W = nl_d2 sum_c gimg_c P_c F_c and the direct
+            // dL/d(n.l) = sum_c gimg_c P_c f_c / d^2
+            const float ndg = s.nl_d2 * s.DGq;
+            float U, gnl;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float pf = Pf1[c] + s.DGq * PF[c];
@@ -337,12 +341,14 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
                 // 2.2 d out/d img = ic^(1/2.2 - 1) inside the clamp, 0 outside
                 // (a NaN img is outside, as torch's clamp backward masks it)
                 const float gimg = ic == img ? go[c] * fexp2(flog2(ic) * (kInvGamma - 1.0f)) : 0.0f;
-                gf[c] = gimg * s.nl_d2;
-                gnl += gimg * pf;
+                a.S[c] += gimg * s.nl_d2;
+                a.T[c] += gimg * ndg;
+                U = c == 0 ? gimg * PF[c] : U + gimg * PF[c];
+                gnl = c == 0 ? gimg * pf : gnl + gimg * pf;
                 // dL/dP_c = sum gimg f_c nl_d2 = (sum gimg img_c) / P_c
                 if (LIGHT_GRAD) a.gP[c] += gimg * img;
             }
-            backward_coloc(m, s, gf, gnl * s.inv_d2, PF, a);
+            coloc_gradients(m, s, U * s.nl_d2, gnl * s.inv_d2, a);
         } else {
             Geom g;
             geometry<false>(px.px, px.py, vp.cx, vp.cy, vp.cz, vp.lx, vp.ly, vp.lz, g);
