@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 9
+#define SVBRDF_ABI_VERSION 10
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -109,7 +109,7 @@ int svbrdf_render_backward_terms(const float *textures, int res, int n_views, fl
 /* Adjoint of svbrdf_render: the backward of Microfacet.eval under torch
  * autograd (src/microfacet.py:64-120).  Given dL/dimages it writes
  * dL/dtextures (overwritten, not accumulated) and, when grad_light_pow is not
- * NULL, dL/dlight_pow[3] (needs a workspace of
+ * NULL, dL/dlight_pow[3] (needs a 16-byte aligned workspace of
  * svbrdf_render_backward_workspace_bytes(res) bytes; otherwise workspace may
  * be NULL).  The BRDF terms are recomputed, not stored. */
 size_t svbrdf_render_backward_workspace_bytes(int res);
@@ -123,7 +123,7 @@ int svbrdf_render_backward(const float *textures, int res, int n_views, float si
  * allocation: no counterpart in the reference.  It is for callers that
  * cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  The light-power partial
- * sums go to a 1 MiB device array inside the library (one per GPU, held
+ * sums go to a 4 MiB device array inside the library (one per GPU, held
  * until the process ends), so res is limited to 4096 (larger res returns
  * SVBRDF_ESHAPE: use svbrdf_render_backward with a workspace).  The first
  * call on a device resolves that array's address (hipGetSymbolAddress, no

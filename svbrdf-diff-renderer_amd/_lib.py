@@ -15,7 +15,7 @@ _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("SVBRDF_AMD_LIB") or _PKG / "libsvbrdf_hip.so")
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 OK = 0
 EINVAL = -1

@@ -32,6 +32,18 @@ using namespace svbrdf;
 namespace {
 
 constexpr int kBlock = 256;
+// The forward render and the adjoint launch one wave per block.  Their waves
+// run long (one pixel per lane through every view), and a workgroup starts on
+// a CU only when all of its waves fit there at once: with 256-thread blocks
+// the slots freed by the first waves of a block to finish stay empty until
+// enough others free up, while one-wave blocks refill each slot as it frees.
+// Measured at 1024^2 x 9, bit-identical results: adjoint 38.6 -> 31.6 us,
+// forward 26.5 -> 24.7 us.  The fused step keeps 256-thread blocks: its
+// per-block arrival atomics and the next step's reduction of the per-block
+// partials grow with the block count (64-thread blocks: 53 -> 88 us; 128: no
+// change).
+constexpr int kRenderBlock = 64;
+constexpr int kAdjointBlock = 64;
 constexpr int kMaxRes = 7680;   // 9 planes of res^2 floats (and every plane offset) below 2^31 bytes
 constexpr int kMaxViews = 1024;
 
@@ -60,9 +72,9 @@ int check_common(const char *what, const void *tex, int res, int n_views, const 
     return SVBRDF_OK;
 }
 
-inline unsigned n_blocks(int res, int npix_per_thread = 1) {
+inline unsigned n_blocks(int res, int npix_per_thread = 1, int block = kBlock) {
     const size_t npix = (size_t)res * res;
-    const size_t per_block = (size_t)kBlock * npix_per_thread;
+    const size_t per_block = (size_t)block * npix_per_thread;
     return (unsigned)((npix + per_block - 1) / per_block);
 }
 
@@ -93,7 +105,7 @@ __device__ __forceinline__ ViewPos load_view(const float *__restrict__ cam, cons
 // trip of vector loads per 64 coordinates at the start of a block (a loop of
 // scalar loads that stops at the first difference waits on 3 V dependent
 // round trips there).  The whole wave must be active: every caller reaches it
-// with all 64 lanes on (full 256-thread blocks, outside any divergent branch).
+// with all 64 lanes on (full waves, outside any divergent branch).
 __device__ __forceinline__ bool all_colocated(const float *__restrict__ cam, const float *__restrict__ light,
                                               int n_views) {
     const int n = 3 * n_views, lane = (int)(threadIdx.x & 63u);
@@ -172,22 +184,26 @@ __device__ __forceinline__ Pix pixel_at(unsigned p, unsigned npix, int res, floa
     return x;
 }
 
-// Wave64 + block reduction of a float; result valid in thread 0.
+// Wave64 + block reduction of a float over BLOCK threads; result valid in
+// thread 0 (a one-wave block needs no LDS round).
+template <int BLOCK = kBlock>
 __device__ __forceinline__ float block_sum(float x, float *lds) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if constexpr (BLOCK == 64) return x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) lds[wave] = x;
     __syncthreads();
     float r = 0.f;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) r += lds[w];
+        for (int w = 0; w < BLOCK / 64; ++w) r += lds[w];
     }
     __syncthreads();
     return r;
 }
 
+// The same for a double over a 256-thread block.
 __device__ __forceinline__ double block_sum_d(double x, double *lds) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
@@ -261,14 +277,14 @@ __device__ __forceinline__ void render_pixels(const Pix (&px)[NPIX], const float
 // TERMS != 0 (alternate BRDF terms) always takes the general path: the
 // co-located specialisation hard-codes GGX and the spherical-Gaussian Fresnel.
 template <int NPIX, int TERMS = 0>
-__global__ __launch_bounds__(kBlock) void render_kernel(
+__global__ __launch_bounds__(kRenderBlock) void render_kernel(
     const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, const float *__restrict__ lpow, float *__restrict__ out) {
     const unsigned npix = (unsigned)res * (unsigned)res;
     Pix px[NPIX];
 #pragma unroll
     for (int q = 0; q < NPIX; ++q)
-        px[q] = pixel_at(blockIdx.x * (kBlock * NPIX) + q * kBlock + threadIdx.x, npix, res, size);
+        px[q] = pixel_at(blockIdx.x * (kRenderBlock * NPIX) + q * kRenderBlock + threadIdx.x, npix, res, size);
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
     if (TERMS == 0 && all_colocated(cam, light, n_views))
         render_pixels<true, NPIX>(px, tex, npix, n_views, cam, light, P, out);
@@ -386,13 +402,13 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
                                      // variants spill up to 12 B, the others none)
 #endif
 template <bool LIGHT_GRAD, int TERMS = 0>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_BACKWARD_WAVES))) void backward_kernel(
+__global__ __launch_bounds__(kAdjointBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_BACKWARD_WAVES))) void backward_kernel(
     const float *__restrict__ tex, int res, int n_views, float size, const float *__restrict__ cam,
     const float *__restrict__ light, const float *__restrict__ lpow, const float *__restrict__ gout,
-    float *__restrict__ gtex, float *__restrict__ light_partials) {
-    __shared__ float lds[kBlock / 64];
+    float *__restrict__ gtex, float4 *__restrict__ light_partials) {
+    __shared__ float lds[kAdjointBlock / 64];
     const unsigned npix = (unsigned)res * (unsigned)res;
-    const Pix px = pixel_at(blockIdx.x * kBlock + threadIdx.x, npix, res, size);
+    const Pix px = pixel_at(blockIdx.x * kAdjointBlock + threadIdx.x, npix, res, size);
     const float P[3] = {lpow[0], lpow[1], lpow[2]};
     Accum a;
     accum_zero(a);
@@ -401,24 +417,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_B
     else
         backward_pixel<false, LIGHT_GRAD, TERMS>(px, tex, npix, n_views, cam, light, P, gout, gtex, a);
     if (LIGHT_GRAD) {
+        float s[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            // a lane past the end shaded a copy of the last pixel: no contribution
-            const float s = block_sum(px.valid ? a.gP[c] : 0.0f, lds);
-            if (threadIdx.x == 0) light_partials[(size_t)blockIdx.x * 4 + c] = s;
-        }
+        for (int c = 0; c < 3; ++c)   // a lane past the end shaded a copy of the last pixel: no contribution
+            s[c] = block_sum<kAdjointBlock>(px.valid ? a.gP[c] : 0.0f, lds);
+        if (threadIdx.x == 0) light_partials[blockIdx.x] = make_float4(s[0], s[1], s[2], 0.0f);
     }
 }
 
-// Sums per-block light-power partials in a fixed order (deterministic).
-__global__ __launch_bounds__(kBlock) void reduce_light_kernel(const float *__restrict__ partials,
-                                                              unsigned n, float *__restrict__ out) {
-    __shared__ double lds[kBlock / 64];
+// Sums the adjoint's per-block light-power partials (float4: dL/dP rgb, 0) in
+// a fixed order (deterministic): thread i adds partials i, i + 1024, ... in
+// double, four loads in flight, then the waves combine in index order.  1,024
+// threads take the 16,384 partials of a 1024^2 adjoint in four rounds.
+constexpr int kLightReduceBlock = 1024;
+__global__ __launch_bounds__(kLightReduceBlock) void reduce_light_kernel(const float4 *__restrict__ partials,
+                                                                         unsigned n, float *__restrict__ out) {
+    __shared__ double lds[3][kLightReduceBlock / 64];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (unsigned b0 = threadIdx.x; b0 < n; b0 += 4 * kLightReduceBlock) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned b = b0 + u * kLightReduceBlock;
+            q[u] = b < n ? partials[b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc[0] += q[u].x;
+            acc[1] += q[u].y;
+            acc[2] += q[u].z;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double acc = 0.0;
-        for (unsigned b = threadIdx.x; b < n; b += kBlock) acc += (double)partials[(size_t)b * 4 + c];
-        const double s = block_sum_d(acc, lds);
-        if (threadIdx.x == 0) out[c] = (float)s;
+        double x = acc[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) lds[c][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double t = 0.0;
+            for (int w = 0; w < kLightReduceBlock / 64; ++w) t += lds[c][w];
+            out[c] = (float)t;
+        }
     }
 }
 
@@ -1184,14 +1229,14 @@ int check_variant(const char *what, int loss_mode, int npix, int unroll) {
 
 // ---------------------------------------------------------------------------
 // Allocation-free adjoint.  The light-power partial sums (one float4 slot per
-// 256-pixel block) of svbrdf_render_backward_noalloc live in this file-scope
-// device array instead of a caller workspace: 65,536 slots, 1 MiB, enough for
+// 64-pixel block) of svbrdf_render_backward_noalloc live in this file-scope
+// device array instead of a caller workspace: 262,144 slots, 4 MiB, enough for
 // res <= 4096.  Each GPU holds its own copy for the life of the process.  In
 // host code `scratch` names a host-side stand-in, not device memory: the
 // device address comes from hipGetSymbolAddress, once per device (with that
 // device current), and only that address reaches kernels.
 constexpr int kScratchMaxRes = 4096;
-constexpr unsigned kScratchSlots = (unsigned)(((size_t)kScratchMaxRes * kScratchMaxRes + kBlock - 1) / kBlock);
+constexpr unsigned kScratchSlots = (unsigned)(((size_t)kScratchMaxRes * kScratchMaxRes + kAdjointBlock - 1) / kAdjointBlock);
 constexpr size_t kScratchBytes = (size_t)kScratchSlots * 4 * sizeof(float);
 }  // namespace
 
@@ -1250,8 +1295,8 @@ int svbrdf_render_terms(const float *textures, int res, int n_views, float size,
 #define SVBRDF_RENDER_NPIX 1      // pixels per thread of the forward render (2 measured equal at 1024^2 x 9)
 #endif
 #define SVBRDF_LAUNCH_RENDER(T)                                                                               \
-    hipLaunchKernelGGL((render_kernel<SVBRDF_RENDER_NPIX, T>), dim3(n_blocks(res, SVBRDF_RENDER_NPIX)),         \
-                       dim3(kBlock), 0, s, textures, res, n_views, size, camera_pos, light_pos, light_pow, images)
+    hipLaunchKernelGGL((render_kernel<SVBRDF_RENDER_NPIX, T>), dim3(n_blocks(res, SVBRDF_RENDER_NPIX, kRenderBlock)),         \
+                       dim3(kRenderBlock), 0, s, textures, res, n_views, size, camera_pos, light_pos, light_pow, images)
     switch (terms) {
         case 0: SVBRDF_LAUNCH_RENDER(0); break;
         case 1: SVBRDF_LAUNCH_RENDER(1); break;
@@ -1270,7 +1315,7 @@ int svbrdf_render(const float *textures, int res, int n_views, float size, const
 
 size_t svbrdf_render_backward_workspace_bytes(int res) {
     if (res < 1 || res > kMaxRes) return 0;
-    return (size_t)n_blocks(res) * 4 * sizeof(float);
+    return (size_t)n_blocks(res, 1, kAdjointBlock) * sizeof(float4);
 }
 
 int svbrdf_render_backward_terms(const float *textures, int res, int n_views, float size, const float *camera_pos,
@@ -1284,16 +1329,18 @@ int svbrdf_render_backward_terms(const float *textures, int res, int n_views, fl
     if (terms < 0 || terms > (SVBRDF_NDF_BECKMANN | SVBRDF_FRESNEL_SCHLICK))
         return fail(SVBRDF_EINVAL, "svbrdf_render_backward: unknown terms %d", terms);
     hipStream_t s = (hipStream_t)stream;
-    const unsigned nb = n_blocks(res);
+    const unsigned nb = n_blocks(res, 1, kAdjointBlock);
     if (grad_light_pow) {
         const size_t need = svbrdf_render_backward_workspace_bytes(res);
         if (!workspace || workspace_bytes < need)
             return fail(SVBRDF_ESPACE, "svbrdf_render_backward: workspace of %zu bytes < %zu needed",
                         workspace_bytes, need);
+        if ((uintptr_t)workspace % 16)
+            return fail(SVBRDF_EINVAL, "svbrdf_render_backward: workspace must be 16-byte aligned");
     }
-    float *parts = grad_light_pow ? (float *)workspace : nullptr;
+    float4 *parts = grad_light_pow ? (float4 *)workspace : nullptr;
 #define SVBRDF_LAUNCH_BWD(L, T)                                                                                \
-    hipLaunchKernelGGL((backward_kernel<L, T>), dim3(nb), dim3(kBlock), 0, s, textures, res, n_views, size,      \
+    hipLaunchKernelGGL((backward_kernel<L, T>), dim3(nb), dim3(kAdjointBlock), 0, s, textures, res, n_views, size,      \
                        camera_pos, light_pos, light_pow, grad_images, grad_textures, parts)
 #define SVBRDF_LAUNCH_BWD_T(L)                                                                                 \
     switch (terms) {                                                                                            \
@@ -1311,8 +1358,8 @@ int svbrdf_render_backward_terms(const float *textures, int res, int n_views, fl
 #undef SVBRDF_LAUNCH_BWD
     if ((rc = check_launch("svbrdf_render_backward"))) return rc;
     if (grad_light_pow)
-        hipLaunchKernelGGL(reduce_light_kernel, dim3(1), dim3(kBlock), 0, s, (const float *)workspace, nb,
-                           grad_light_pow);
+        hipLaunchKernelGGL(reduce_light_kernel, dim3(1), dim3(kLightReduceBlock), 0, s, (const float4 *)workspace,
+                           nb, grad_light_pow);
     return check_launch("svbrdf_render_backward");
 }
 

@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 9 == pkg._lib.ABI_VERSION
+    assert lib.svbrdf_abi_version() == 10 == pkg._lib.ABI_VERSION
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
@@ -46,7 +46,9 @@ def test_queries(lib, pkg):
     assert lib.svbrdf_fused_step_workspace_bytes(1024) == 8448 + 2 * (1024 * 1024 // 256 * 16)
     assert lib.svbrdf_fused_step_workspace_bytes(1) == 8448 + 2 * 256
     assert lib.svbrdf_fused_step_workspace_bytes(0) == 0
-    assert lib.svbrdf_render_backward_workspace_bytes(13) == 1 * 16
+    # one float4 light-power partial per 64-pixel block of the adjoint
+    assert lib.svbrdf_render_backward_workspace_bytes(13) == 3 * 16
+    assert lib.svbrdf_render_backward_workspace_bytes(1024) == 1024 * 1024 // 64 * 16
     # padded plane stride: res^2 rounded up to 64 floats, plus 448
     assert lib.svbrdf_plane_stride(1024) == 1024 * 1024 + 448
     assert lib.svbrdf_plane_stride(1023) == 1046592 + 448 and 1046592 >= 1023 * 1023
@@ -78,6 +80,13 @@ def test_backward_requires_workspace_for_light_gradient(lib):
                                     _fake(6), _fake(7), None, 0, None)
     assert rc == -3
     assert b"workspace" in lib.svbrdf_last_error()
+    # one float4 partial per 64-pixel block, 16-byte aligned
+    need = lib.svbrdf_render_backward_workspace_bytes(64)
+    assert need == 64 * 64 // 64 * 16
+    rc = lib.svbrdf_render_backward(_fake(1), 64, 9, ctypes.c_float(17.0), _fake(2), _fake(3), _fake(4), _fake(5),
+                                    _fake(6), _fake(7), ctypes.c_void_p(0x10008), need, None)
+    assert rc == -1
+    assert b"16-byte aligned" in lib.svbrdf_last_error()
 
 
 def _step(lib, **kw):
@@ -158,7 +167,7 @@ def test_sizes_beyond_the_limits_are_refused(lib):
 
 
 def test_noalloc_adjoint_refuses_sizes_beyond_its_scratch(lib):
-    """svbrdf_render_backward_noalloc serves res <= 4096 (its 1 MiB array);
+    """svbrdf_render_backward_noalloc serves res <= 4096 (its 4 MiB array);
     larger sizes are refused on the host before any HIP call."""
     f = _fake()
     assert lib.svbrdf_render_backward_noalloc(f, 4097, 9, 17.0, f, f, f, f, f, f, None) == -2
