@@ -136,6 +136,29 @@ def test_adjoint_1024_band(pkg):
     parity.check_gradient(dtex[:, rows], g32, g64, "1024 band")
 
 
+def test_adjoint_light_gradient_1024(pkg):
+    """dL/dP at 1024^2 x 9 views through all 16,384 one-wave blocks' partials
+    and the fixed-order reduction.  dL/dimages is zero outside the first and
+    last 64 rows, so the FP64 oracle needs only those rows; every block still
+    writes a partial (zero between the bands)."""
+    res, V = 1024, 9
+    tex, _, size, cam, light, P = scene(pkg, res, V, seed=1)
+    rng = np.random.default_rng(11)
+    gout = np.zeros((V, 3, res, res), np.float32)
+    bands = (slice(0, 64), slice(res - 64, res))
+    for b in bands:
+        gout[:, :, b] = rng.standard_normal((V, 3, 64, res)).astype(np.float32)
+    _, dtex, dP = gpu_render(pkg, tex, res, size, cam, light, P, gout=gout)
+    dP64 = np.zeros(3)
+    for b in bands:
+        g64, p64 = M.render_vjp(tex, res, size, cam, light, P, gout[:, :, b], np.float64, rows=b)
+        g32, _ = M.render_vjp(tex, res, size, cam, light, P, gout[:, :, b], np.float32, rows=b)
+        parity.check_gradient(dtex[:, b], g32, g64, f"1024 rows {b.start}-{b.stop}")
+        dP64 += p64
+    assert np.all(dtex[:, 64:res - 64] == 0.0)
+    np.testing.assert_allclose(dP, dP64, rtol=1e-4)
+
+
 def test_eval_is_deterministic_and_stream_ordered(pkg):
     args = scene(pkg, 128, 9, seed=3)
     a = gpu_render(pkg, *args)
