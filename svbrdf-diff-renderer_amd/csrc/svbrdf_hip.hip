@@ -37,7 +37,7 @@ constexpr int kBlock = 256;
 // a CU only when all of its waves fit there at once: with 256-thread blocks
 // the slots freed by the first waves of a block to finish stay empty until
 // enough others free up, while one-wave blocks refill each slot as it frees.
-// Measured at 1024^2 x 9, bit-identical results: adjoint 38.6 -> 31.6 us,
+// Measured at 1024^2 x 9, bit-identical results: adjoint 38.5 -> 31.2 us,
 // forward 26.5 -> 24.7 us.  The fused step keeps 256-thread blocks: its
 // per-block arrival atomics and the next step's reduction of the per-block
 // partials grow with the block count (64-thread blocks: 53 -> 88 us; 128: no
@@ -398,8 +398,8 @@ __device__ __forceinline__ void backward_pixel(const Pix &px, const float *__res
 }
 
 #ifndef SVBRDF_BACKWARD_WAVES
-#define SVBRDF_BACKWARD_WAVES 5      // minimum waves per SIMD for the adjoint (93-96 VGPRs; the light-power
-                                     // variants spill up to 12 B, the others none)
+#define SVBRDF_BACKWARD_WAVES 5      // minimum waves per SIMD for the adjoint (91-96 VGPRs; 8-28 B of spills in
+                                     // some variants, none inside the view loops)
 #endif
 template <bool LIGHT_GRAD, int TERMS = 0>
 __global__ __launch_bounds__(kAdjointBlock) __attribute__((amdgpu_waves_per_eu(SVBRDF_BACKWARD_WAVES))) void backward_kernel(
@@ -1295,8 +1295,9 @@ int svbrdf_render_terms(const float *textures, int res, int n_views, float size,
 #define SVBRDF_RENDER_NPIX 1      // pixels per thread of the forward render (2 measured equal at 1024^2 x 9)
 #endif
 #define SVBRDF_LAUNCH_RENDER(T)                                                                               \
-    hipLaunchKernelGGL((render_kernel<SVBRDF_RENDER_NPIX, T>), dim3(n_blocks(res, SVBRDF_RENDER_NPIX, kRenderBlock)),         \
-                       dim3(kRenderBlock), 0, s, textures, res, n_views, size, camera_pos, light_pos, light_pow, images)
+    hipLaunchKernelGGL((render_kernel<SVBRDF_RENDER_NPIX, T>), dim3(n_blocks(res, SVBRDF_RENDER_NPIX, kRenderBlock)), \
+                       dim3(kRenderBlock), 0, s, textures, res, n_views, size, camera_pos, light_pos, light_pow,       \
+                       images)
     switch (terms) {
         case 0: SVBRDF_LAUNCH_RENDER(0); break;
         case 1: SVBRDF_LAUNCH_RENDER(1); break;
