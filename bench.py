@@ -108,6 +108,7 @@ def algorithmic_bytes_render(res, views):
 
 
 INFINITY_CACHE_BYTES = 256 * 2**20   # MI355X last-level (memory-side) cache
+WARM_KERNEL_LAUNCHES = 1500           # untimed launches before the forward / adjoint legs are timed
 
 
 def working_set_bytes(fs, views):
@@ -935,10 +936,12 @@ def main():
     # forward render at the same shape (1024^2 x 9 views)
     r = prob["renderer"]
     gt = torch.tensor(prob["gt"], device=dev)[None]
-    # 50 untimed launches first: these kernels run below a second, and the
-    # clock settles over the first tens of launches
+    # untimed launches first: the clock settles over the first ~1,000
+    # back-to-back launches of these kernels (a 1,500-launch trace of the
+    # adjoint falls from 38 to 30.6 us per launch, then holds; the forward
+    # render holds from the start)
     with torch.no_grad():
-        for _ in range(50):
+        for _ in range(WARM_KERNEL_LAUNCHES):
             r.eval(gt)
         fms = time_events(lambda: r.eval(gt), 200)
     fb = algorithmic_bytes_render(res, views)
@@ -956,7 +959,7 @@ def main():
     def adjoint():
         L.call("svbrdf_render_backward_terms", L.ptr(gt), res, views, r.size, L.ptr(r.camera_pos),
                L.ptr(r.light_pos), L.ptr(r.light_pow), 0, L.ptr(gout), L.ptr(gtex), None, None, 0, stream)
-    for _ in range(50):
+    for _ in range(WARM_KERNEL_LAUNCHES):
         adjoint()
     bms = time_events(adjoint, 200)
     bb = algorithmic_bytes_adjoint(res, views)
