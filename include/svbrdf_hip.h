@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 10
+#define SVBRDF_ABI_VERSION 11
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -372,6 +372,17 @@ enum svbrdf_activation {
  * (cin and cout exchanged) is dL/dx. */
 int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *style, float scale, int demodulate,
                         int transpose, float *packed, void *stream);
+
+/* dL/dstyle of the weights svbrdf_conv3x3_pack writes with a style and
+ * scale (and demodulation when demodulate != 0), given grad_weight = dL/d(the
+ * packed weight) in the natural [cout, cin, 3, 3] layout (what
+ * svbrdf_conv3x3_wgrad returns): the backward of the modulation and
+ * demodulation of ModulateConvBlock.forward, stylegan2_generator_network.py:
+ * 694-706, which the reference leaves to torch autograd.  grad_style [cin] is
+ * overwritten.  workspace: 8-byte aligned, at least 2 cout floats. */
+int svbrdf_conv3x3_modulation_grad(const float *weight, int cout, int cin, const float *style, float scale,
+                                   int demodulate, const float *grad_weight, float *grad_style, void *workspace,
+                                   size_t workspace_bytes, void *stream);
 
 /* y = act(conv3x3(x, w) + noise_strength * noise + bias), stride 1, zero
  * padding 1 (F.conv2d(x, w, padding=1) as the reference's conv layers call

@@ -162,6 +162,80 @@ def conv3x3(x, weight, bias=None, noise=None, noise_strength=0.0, act="none"):
     return Conv3x3Function.apply(x, weight, bias, noise, noise_strength, act)
 
 
+def modulation_grad(weight, style, scale, demodulate, grad_weight):
+    """dL/dstyle [Cin] of the weights pack3x3(weight, style, scale,
+    demodulate) packs, from dL/d(those weights) in [Cout, Cin, 3, 3]."""
+    w = weight.detach().contiguous()
+    g = grad_weight.contiguous()
+    st = style.detach().reshape(-1).contiguous()
+    for t, name in ((w, "weight"), (g, "grad_weight"), (st, "style")):
+        _check(t, name)
+    cout, cin = int(w.shape[0]), int(w.shape[1])
+    if tuple(g.shape) != tuple(w.shape) or st.numel() != cin:
+        raise ValueError(f"grad_weight {tuple(g.shape)} / style {st.numel()} do not match weight {tuple(w.shape)}")
+    ws = torch.empty(2 * cout, dtype=torch.float32, device=w.device)
+    out = torch.empty(cin, dtype=torch.float32, device=w.device)
+    _lib.call("svbrdf_conv3x3_modulation_grad", _lib.ptr(w), cout, cin, _lib.ptr(st), float(scale),
+              int(bool(demodulate)), _lib.ptr(g), _lib.ptr(out), _lib.ptr(ws), ws.numel() * 4,
+              _lib.stream_handle(w.device))
+    return out
+
+
+class ModConv3x3Function(torch.autograd.Function):
+    """The modulated 3x3 convolution of a StyleGAN2 block (batch 1):
+    y = act(conv3x3(x, w') + noise_strength * noise + bias) with
+    w'[co][ci] = scale style[ci] w[co][ci], divided per output channel by
+    sqrt(sum w'^2 + 1e-8) when demodulate (ModulateConvBlock.forward,
+    src/higan_models/stylegan2_generator_network.py:694-755).  The modulation
+    and demodulation happen inside the weight packing; the backward gives
+    dL/dx, dL/dstyle (through svbrdf_conv3x3_modulation_grad) and dL/dnoise.
+    The base weight w is frozen (the caller optimises latents and noise)."""
+
+    @staticmethod
+    def forward(ctx, x, style, weight, scale, demodulate, bias, noise, noise_strength, act):
+        if weight.requires_grad:
+            raise ValueError("ModConv3x3Function keeps the base weight frozen; use conv3x3 with explicitly "
+                             "modulated weights to train it")
+        packed = pack3x3(weight, style, scale, demodulate)
+        y = conv3x3_packed(x, packed, int(weight.shape[0]), bias, noise, noise_strength, act)
+        ctx.save_for_backward(x, style, weight, noise, y)
+        ctx.scale, ctx.demodulate = float(scale), bool(demodulate)
+        ctx.noise_strength, ctx.act = float(noise_strength), act
+        return y
+
+    @staticmethod
+    def backward(ctx, grad_y):
+        x, style, weight, noise, y = ctx.saved_tensors
+        gx = gstyle = gnoise = None
+        cin = int(weight.shape[1])
+        need_gx, need_gs = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        need_gn = ctx.needs_input_grad[6] and noise is not None
+        if need_gx:
+            packed_t = pack3x3(weight, style, ctx.scale, ctx.demodulate, transpose=True)
+        if not (need_gs or need_gn):
+            if need_gx:
+                gx = conv3x3_input_grad(grad_y, y, packed_t, cin, ctx.act)
+            return gx, None, None, None, None, None, None, None, None
+        gz = _act_grad(grad_y.contiguous(), y, ctx.act).contiguous()
+        if need_gx:
+            gx = conv3x3_packed(gz, packed_t, cin)
+        if need_gs:
+            gw = conv3x3_wgrad(x, gz, int(weight.shape[0]))
+            gstyle = modulation_grad(weight, style, ctx.scale, ctx.demodulate, gw).reshape(style.shape)
+        if need_gn:
+            gnoise = (ctx.noise_strength * gz.sum(dim=(0, 1))).reshape(noise.shape)
+        return gx, gstyle, None, None, None, None, gnoise, None, None
+
+
+def modulated_conv3x3(x, style, weight, scale, demodulate=True, bias=None, noise=None, noise_strength=0.0,
+                      act="none"):
+    if act not in ACT:
+        raise ValueError(f"act must be one of {sorted(ACT)}")
+    if x.shape[0] != 1:
+        raise ValueError("modulated_conv3x3 takes one sample (the style is per input channel)")
+    return ModConv3x3Function.apply(x, style, weight, scale, demodulate, bias, noise, noise_strength, act)
+
+
 class Upsample2Function(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x):

@@ -101,6 +101,72 @@ __global__ __launch_bounds__(kThreads) void conv3x3_pack_kernel(const float *__r
     }
 }
 
+// Gradient of the modulated, demodulated weights with respect to the style:
+// with u = scale style[ci] w[co][ci][t], d_co = 1/sqrt(sum_{ci,t} u^2 + 1e-8)
+// and the packed weight u d_co, given g = dL/d(u d_co) in [Cout, Cin, 3, 3],
+//   dL/du = g d_co - u d_co^3 sum_{ci,t} g u        (e_co = d_co^3 sum g u)
+//   dL/dstyle[ci] = scale sum_{co,t} w (g d_co - e_co u)
+// (d = 1, e = 0 without demodulation).  Two passes with fixed summation
+// orders: one block per output channel for (d, e), one per input channel for
+// the style gradient.
+template <int BLOCK>
+__device__ __forceinline__ float block_sum_f(float v, float *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = 0.0f;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < BLOCK / 64; ++k) r += red[k];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kThreads) void modulation_rows_kernel(const float *__restrict__ w, int cin,
+                                                                   const float *__restrict__ style, float scale,
+                                                                   int demodulate, const float *__restrict__ g,
+                                                                   float2 *__restrict__ de) {
+    __shared__ float red[kThreads / 64];
+    const int co = blockIdx.x, n = cin * 9;
+    const float *wc = w + (size_t)co * n, *gc = g + (size_t)co * n;
+    float ss = 0.0f, gu = 0.0f;
+    for (int e = threadIdx.x; e < n; e += kThreads) {
+        const float u = wc[e] * scale * style[e / 9];
+        ss += u * u;
+        gu += gc[e] * u;
+    }
+    ss = block_sum_f<kThreads>(ss, red);
+    gu = block_sum_f<kThreads>(gu, red);
+    if (threadIdx.x == 0) {
+        const float d = demodulate ? rsqrtf(ss + 1e-8f) : 1.0f;
+        de[co] = make_float2(d, demodulate ? d * d * d * gu : 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void modulation_cols_kernel(const float *__restrict__ w, int cout, int cin,
+                                                                   const float *__restrict__ style, float scale,
+                                                                   const float *__restrict__ g,
+                                                                   const float2 *__restrict__ de,
+                                                                   float *__restrict__ grad_style) {
+    __shared__ float red[kThreads / 64];
+    const int ci = blockIdx.x;
+    const float sc = scale * style[ci];
+    float acc = 0.0f;
+    for (int co = threadIdx.x; co < cout; co += kThreads) {
+        const size_t base = ((size_t)co * cin + ci) * 9;
+        const float2 f = de[co];
+        float r = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const float wv = w[base + t];
+            r += wv * (g[base + t] * f.x - f.y * sc * wv);
+        }
+        acc += r;
+    }
+    acc = block_sum_f<kThreads>(acc, red);
+    if (threadIdx.x == 0) grad_style[ci] = scale * acc;
+}
+
 // ---------------------------------------------------------------------------
 // The convolution.
 struct ConvArgs {
@@ -823,6 +889,29 @@ int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *sty
         return fail(SVBRDF_ESHAPE, "%s: cout=%d cin=%d outside [1, %d]", what, cout, cin, kMaxChannels);
     hipLaunchKernelGGL(conv3x3_pack_kernel, dim3((unsigned)cout), dim3(kThreads), 0, (hipStream_t)stream, weight,
                        cout, cin, style, scale, demodulate ? 1 : 0, transpose ? 1 : 0, packed);
+    return check_launch(what);
+}
+
+int svbrdf_conv3x3_modulation_grad(const float *weight, int cout, int cin, const float *style, float scale,
+                                   int demodulate, const float *grad_weight, float *grad_style, void *workspace,
+                                   size_t workspace_bytes, void *stream) {
+    const char *what = "svbrdf_conv3x3_modulation_grad";
+    if (!weight || !style || !grad_weight || !grad_style || !workspace)
+        return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (cout < 1 || cin < 1 || cout > kMaxChannels || cin > kMaxChannels)
+        return fail(SVBRDF_ESHAPE, "%s: cout=%d cin=%d outside [1, %d]", what, cout, cin, kMaxChannels);
+    if (workspace_bytes < (size_t)cout * sizeof(float2))
+        return fail(SVBRDF_ESPACE, "%s: workspace of %zu bytes < %zu needed", what, workspace_bytes,
+                    (size_t)cout * sizeof(float2));
+    if ((uintptr_t)workspace % sizeof(float2))
+        return fail(SVBRDF_EINVAL, "%s: workspace must be 8-byte aligned", what);
+    hipStream_t s = (hipStream_t)stream;
+    float2 *de = static_cast<float2 *>(workspace);
+    hipLaunchKernelGGL(modulation_rows_kernel, dim3((unsigned)cout), dim3(kThreads), 0, s, weight, cin, style, scale,
+                       demodulate ? 1 : 0, grad_weight, de);
+    if (int rc = check_launch(what)) return rc;
+    hipLaunchKernelGGL(modulation_cols_kernel, dim3((unsigned)cin), dim3(kThreads), 0, s, weight, cout, cin, style,
+                       scale, grad_weight, (const float2 *)de, grad_style);
     return check_launch(what);
 }
 

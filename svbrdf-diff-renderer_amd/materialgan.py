@@ -30,7 +30,9 @@ weights:
 The 3x3 convolutions of both networks run on the matrix-core kernels of
 :mod:`svbrdf_amd.convnet` (forward, input gradient and the generator's
 weight gradient, which the latent's gradient needs), with the noise, bias
-and leaky-ReLU epilogue fused; the x2 upsampling is a HIP kernel too.  The
+and leaky-ReLU epilogue fused; in the generator the style modulation and the
+demodulation happen inside the HIP weight packing, and dL/dstyle comes from
+the HIP modulation backward.  The x2 upsampling is a HIP kernel too.  The
 renderer is :class:`svbrdf_amd.Microfacet` (forward and adjoint kernels).
 The 1x1 toRGB convolutions (9 output channels: plain GEMMs, torch.matmul)
 and the fully connected mapping layers stay torch operations.
@@ -90,14 +92,19 @@ class _ModConv(torch.nn.Module):
         return wt
 
     def forward(self, x, w, bias=None, noise=None, noise_strength=0.0, act="none"):
-        wt = self.modulated_weight(w)
         if self.k == 1:
             # a 1x1 convolution is a plain GEMM: [Cout, Cin] x [N, Cin, H W]
+            wt = self.modulated_weight(w)
             n, c, h, wd = x.shape
             return torch.matmul(wt.view(wt.shape[0], c), x.reshape(n, c, h * wd)).view(n, -1, h, wd)
+        # 3x3: the modulation and demodulation happen inside the HIP weight
+        # packing, and the style's gradient comes from the HIP modulation
+        # backward, instead of torch ops on the whole weight tensor
+        style = self.affine(w)
         if self.upsample:
             x = convnet.upsample2(x)
-        return convnet.conv3x3(x, wt, bias, noise, noise_strength, act)
+        return convnet.modulated_conv3x3(x, style, self.weight, self.scale, self.demodulate, bias, noise,
+                                         noise_strength, act)
 
 
 class StyleGAN2Generator(torch.nn.Module):
@@ -259,8 +266,8 @@ class MaterialGANOptim:
             return self.loss_small_feature(predicts)
         raise ValueError("a feature-loss flag is needed, 'L' or 'N'")
 
-    def latent_to_textures(self, latent):
-        t = self.net_obj.synthesis(latent, self.noises)
+    def latent_to_textures(self, latent, noises=None):
+        t = self.net_obj.synthesis(latent, self.noises if noises is None else noises)
         textures = t.clone()
         textures[:, 0:5] = t[:, 0:5].clamp(-1, 1)
         textures[:, 5] = t[:, 5].clamp(-0.5, 0.5)
@@ -296,7 +303,12 @@ class MaterialGANOptim:
         params = [self.latent] if which == "L" else list(self.noises)
         for q in params:
             q.grad = None                                            # optimizer.zero_grad()
-        textures = self.latent_to_textures(self.latent)
+        # the epoch's Adam steps only its own parameters: the other kind enters
+        # detached, so no gradient is computed for it (the reference's
+        # autograd computes one and discards it)
+        latent = self.latent if which == "L" else self.latent.detach()
+        noises = list(self.noises) if which == "N" else [n.detach() for n in self.noises]
+        textures = self.latent_to_textures(latent, noises)
         rendereds = self.renderer_obj.eval(textures)
         loss_image = self.compute_image_loss(rendereds)
         loss_feature = self.compute_feature_loss(rendereds, which) * 0.1
@@ -373,7 +385,9 @@ class MaterialGANOptim:
                     params = params + [light]
                     self.renderer_obj.update_light(light)
                 optimizer = torch.optim.Adam(params, lr=lr, betas=(0.9, 0.999))
-                textures = self.latent_to_textures(self.latent)
+                latent = self.latent if which == "L" else self.latent.detach()
+                noises = list(self.noises) if which == "N" else [n.detach() for n in self.noises]
+                textures = self.latent_to_textures(latent, noises)
                 rendereds = self.renderer_obj.eval(textures)
                 loss_image = self.compute_image_loss(rendereds)
                 loss_feature = self.compute_feature_loss(rendereds, which) * 0.1

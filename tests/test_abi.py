@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 10 == pkg._lib.ABI_VERSION
+    assert lib.svbrdf_abi_version() == 11 == pkg._lib.ABI_VERSION
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096

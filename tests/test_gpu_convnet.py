@@ -12,6 +12,7 @@ side of a kink from FP64 at a handful of pre-activations within rounding of
 gradient by a finite step; there the FP64 gradient is taken with the FP32
 forward's branches, and the flips themselves are bounded (at most 1e-6 of
 the activations, each within 1e-4 of its layer's largest pre-activation)."""
+import math
 import numpy as np
 import pytest
 import torch
@@ -255,6 +256,46 @@ def _gradient_check(g, g64m, g64, g32, label):
     e32 = float((g32.double() - g64).norm() / g64.norm())
     print(f"{label}: |g - g64(same branches)|/|g64| {e:.2e}; |g - g64| {e_all:.2e} (torch FP32 {e32:.2e})")
     assert e <= 1e-5, (label, e)
+
+
+@pytest.mark.parametrize("demod,cin,cout,side", [(True, 64, 32, 16), (False, 32, 64, 8), (True, 512, 512, 4),
+                                                (True, 128, 256, 32)])
+def test_modulated_conv3x3_against_torch(pkg, demod, cin, cout, side):
+    """modulated_conv3x3 -- the style modulation and demodulation inside the
+    HIP weight packing, dL/dstyle from svbrdf_conv3x3_modulation_grad --
+    against the torch restatement of ModulateConvBlock (weights scaled by the
+    style, demodulated, then F.conv2d + noise + bias) in FP64: the output,
+    dL/dx, dL/dstyle and dL/dnoise (no activation, so no branch kinks)."""
+    from svbrdf_amd import convnet
+    g = torch.Generator().manual_seed(cin + cout + side)
+    x = torch.randn(1, cin, side, side, generator=g)
+    style = torch.randn(1, cin, generator=g) * 0.5 + 1.0
+    w = torch.randn(cout, cin, 3, 3, generator=g)
+    bias = torch.randn(cout, generator=g) * 0.1
+    noise = torch.randn(1, 1, side, side, generator=g)
+    probe = torch.randn(1, cout, side, side, generator=g)
+    scale = 1.0 / math.sqrt(cin * 9)
+
+    def ref(xx, ss, nn):
+        wt = w.double() * scale * ss.view(1, -1, 1, 1)
+        if demod:
+            wt = wt * torch.rsqrt((wt * wt).sum(dim=(1, 2, 3), keepdim=True) + 1e-8)
+        return R.conv3x3_ref(xx, wt, bias.double(), nn, 0.3, "none")
+
+    xs, ss, ns = (t.to(DEV).requires_grad_(True) for t in (x, style, noise))
+    y = convnet.modulated_conv3x3(xs, ss, w.to(DEV), scale, demod, bias.to(DEV), ns, 0.3, "none")
+    (y * probe.to(DEV)).sum().backward()
+    x64, s64, n64 = (t.double().requires_grad_(True) for t in (x, style, noise))
+    y64 = ref(x64, s64, n64)
+    (y64 * probe.double()).sum().backward()
+    torch.cuda.synchronize()
+    ey = float((y.detach().cpu().double() - y64.detach()).abs().max() / y64.abs().max())
+    assert ey <= 1e-5, ey
+    for got, want, name in ((xs.grad, x64.grad, "dL/dx"), (ss.grad, s64.grad, "dL/dstyle"),
+                            (ns.grad, n64.grad, "dL/dnoise")):
+        e = float((got.cpu().double() - want).norm() / want.norm())
+        print(f"{name}: relative error {e:.2e}")
+        assert e <= 1e-5, (name, e)
 
 
 def test_generator_synthesis_and_gradients(pkg):
