@@ -214,3 +214,16 @@ def test_conv_weight_gradient_plan_and_validation(lib):
     assert lib.svbrdf_avgpool2_adjoint(None, 1, 4, 4, f, None) == -1
     assert lib.svbrdf_conv3x3_input_grad(f, f, 9, 1, 8, 4, 4, f, 8, f, None, 0, None) == -1   # activation
     assert lib.svbrdf_conv3x3_input_grad(f, None, 1, 1, 8, 4, 4, f, 8, f, None, 0, None) == -1
+
+
+def test_modulation_grad_validation(lib):
+    """svbrdf_conv3x3_modulation_grad refuses null pointers, bad channel counts
+    and a workspace shorter than 2 cout floats (or misaligned) before any
+    launch."""
+    f = _fake()
+    assert lib.svbrdf_conv3x3_modulation_grad(None, 8, 8, f, 1.0, 1, f, f, f, 64, None) == -1
+    assert lib.svbrdf_conv3x3_modulation_grad(f, 8, 8, None, 1.0, 1, f, f, f, 64, None) == -1
+    assert lib.svbrdf_conv3x3_modulation_grad(f, 0, 8, f, 1.0, 1, f, f, f, 64, None) == -2
+    assert lib.svbrdf_conv3x3_modulation_grad(f, 8, 8, f, 1.0, 1, f, f, f, 63, None) == -3
+    assert lib.svbrdf_conv3x3_modulation_grad(f, 8, 8, f, 1.0, 1, f, f, ctypes.c_void_p(0x10004), 64, None) == -1
+    assert b"aligned" in lib.svbrdf_last_error()
