@@ -65,39 +65,102 @@ constexpr int kMaxChannels = 4096;
 constexpr int kMaxSide = 8192;
 
 // ---------------------------------------------------------------------------
-// Weight packing.  One block per output channel: the demodulation factor is a
-// block reduction over its Cin x 9 weights.
+// Weight packing.  The demodulation factor of an output channel is a block
+// reduction over its Cin x 9 scaled weights, in the same per-thread order in
+// every form below, so the factors and the packed values are the same bits.
+// The packed layouts hold consecutive output channels ([9][Cin][Cout]) or,
+// for the input gradient, consecutive input channels ([9][Cout][Cin]) side by
+// side; where that count is a multiple of 4 a thread writes four of them as
+// one 16-byte store (a block per 4 output channels, or per output channel
+// with 4 input channels per step) instead of 4-byte stores a row apart.
+__device__ float pack_factor(const float *__restrict__ wc, int n, const float *__restrict__ style, float scale,
+                             float *red) {
+    float acc = 0.0f;
+    for (int e = threadIdx.x; e < n; e += kThreads) {
+        const float v = wc[e] * scale * (style ? style[e / 9] : 1.0f);
+        acc += v * v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    const float f = rsqrtf(((red[0] + red[1]) + red[2]) + red[3] + 1e-8f);
+    __syncthreads();
+    return f;
+}
+
+// MODE 0: one block per output channel, any shape, 4-byte stores (either
+//         layout, by `transpose`).
+// MODE 1: [9][Cin][Cout], one block per 4 output channels (Cout % 4 == 0).
+// MODE 2: [9][Cout][Cin] with the taps flipped, one block per output channel,
+//         4 input channels per step (Cin % 4 == 0).
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void conv3x3_pack_kernel(const float *__restrict__ w, int cout, int cin,
                                                                 const float *__restrict__ style, float scale,
                                                                 int demodulate, int transpose,
                                                                 float *__restrict__ packed) {
     __shared__ float red[kThreads / 64];
-    __shared__ float s_factor;
-    const int co = blockIdx.x;
     const int n = cin * 9;
-    const float *wc = w + (size_t)co * n;
-    float factor = 1.0f;
-    if (demodulate) {
-        float acc = 0.0f;
-        for (int e = threadIdx.x; e < n; e += kThreads) {
-            const float v = wc[e] * scale * (style ? style[e / 9] : 1.0f);
-            acc += v * v;
-        }
+    if constexpr (MODE == 1) {
+        const int co0 = blockIdx.x * 4;
+        float factor[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        if (demodulate) {
+            // the four channels' sums in one pass, each in pack_factor's order
+            __shared__ float red4[4][kThreads / 64];
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+            for (int e = threadIdx.x; e < n; e += kThreads) {
+                const float sv = style ? style[e / 9] : 1.0f;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) s_factor = rsqrtf(((red[0] + red[1]) + red[2]) + red[3] + 1e-8f);
-        __syncthreads();
-        factor = s_factor;
-    }
-    for (int e = threadIdx.x; e < n; e += kThreads) {
-        const int ci = e / 9, t = e - ci * 9;
-        const float v = wc[e] * scale * (style ? style[ci] : 1.0f) * factor;
-        if (transpose)
-            packed[((size_t)(8 - t) * cout + co) * cin + ci] = v;   // [9][Cout][Cin], taps flipped
-        else
-            packed[((size_t)t * cin + ci) * cout + co] = v;         // [9][Cin][Cout]
+                for (int j = 0; j < 4; ++j) {
+                    const float v = w[(size_t)(co0 + j) * n + e] * scale * sv;
+                    acc[j] += v * v;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+                if ((threadIdx.x & 63) == 0) red4[j][threadIdx.x >> 6] = acc[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                factor[j] = rsqrtf(((red4[j][0] + red4[j][1]) + red4[j][2]) + red4[j][3] + 1e-8f);
+        }
+#pragma unroll 2
+        for (int e = threadIdx.x; e < n; e += kThreads) {
+            const int ci = e / 9, t = e - ci * 9;
+            const float sv = style ? style[ci] : 1.0f;
+            float v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = w[(size_t)(co0 + j) * n + e] * scale * sv * factor[j];
+            *reinterpret_cast<float4 *>(packed + ((size_t)t * cin + ci) * cout + co0) = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    } else {
+        const int co = blockIdx.x;
+        const float *wc = w + (size_t)co * n;
+        const float factor = demodulate ? pack_factor(wc, n, style, scale, red) : 1.0f;
+        if constexpr (MODE == 2) {
+            for (int q = threadIdx.x; q < n / 4; q += kThreads) {
+                const int ci0 = (q / 9) * 4, t = q - (q / 9) * 9;
+                float v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    v[j] = wc[(ci0 + j) * 9 + t] * scale * (style ? style[ci0 + j] : 1.0f) * factor;
+                *reinterpret_cast<float4 *>(packed + ((size_t)(8 - t) * cout + co) * cin + ci0) =
+                    make_float4(v[0], v[1], v[2], v[3]);
+            }
+        } else {
+            for (int e = threadIdx.x; e < n; e += kThreads) {
+                const int ci = e / 9, t = e - ci * 9;
+                const float v = wc[e] * scale * (style ? style[ci] : 1.0f) * factor;
+                if (transpose)
+                    packed[((size_t)(8 - t) * cout + co) * cin + ci] = v;   // [9][Cout][Cin], taps flipped
+                else
+                    packed[((size_t)t * cin + ci) * cout + co] = v;         // [9][Cin][Cout]
+            }
+        }
     }
 }
 
@@ -887,8 +950,18 @@ int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *sty
     if (!weight || !packed) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
     if (cout < 1 || cin < 1 || cout > kMaxChannels || cin > kMaxChannels)
         return fail(SVBRDF_ESHAPE, "%s: cout=%d cin=%d outside [1, %d]", what, cout, cin, kMaxChannels);
-    hipLaunchKernelGGL(conv3x3_pack_kernel, dim3((unsigned)cout), dim3(kThreads), 0, (hipStream_t)stream, weight,
-                       cout, cin, style, scale, demodulate ? 1 : 0, transpose ? 1 : 0, packed);
+    hipStream_t s = (hipStream_t)stream;
+    const int dm = demodulate ? 1 : 0, tr = transpose ? 1 : 0;
+    const bool aligned = ((uintptr_t)packed & 15u) == 0;
+    if (!transpose && aligned && cout % 4 == 0)
+        hipLaunchKernelGGL(conv3x3_pack_kernel<1>, dim3((unsigned)(cout / 4)), dim3(kThreads), 0, s, weight, cout, cin,
+                           style, scale, dm, tr, packed);
+    else if (transpose && aligned && cin % 4 == 0)
+        hipLaunchKernelGGL(conv3x3_pack_kernel<2>, dim3((unsigned)cout), dim3(kThreads), 0, s, weight, cout, cin, style,
+                           scale, dm, tr, packed);
+    else
+        hipLaunchKernelGGL(conv3x3_pack_kernel<0>, dim3((unsigned)cout), dim3(kThreads), 0, s, weight, cout, cin, style,
+                           scale, dm, tr, packed);
     return check_launch(what);
 }
 
