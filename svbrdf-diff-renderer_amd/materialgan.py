@@ -66,7 +66,12 @@ class _Dense(torch.nn.Module):
         self.lr_mul = lr_mul
 
     def forward(self, x):
-        return F.linear(x, self.weight * self.scale, self.bias * self.lr_mul)
+        # x W^T scale + b lr_mul as one GEMM with alpha (no pass over the
+        # weight to scale it); lr_mul is 1 in the style layers
+        b = self.bias if self.lr_mul == 1.0 else self.bias * self.lr_mul
+        if x.dim() == 2:
+            return torch.addmm(b, x, self.weight.t(), alpha=self.scale)
+        return F.linear(x, self.weight * self.scale, b)
 
 
 class _ModConv(torch.nn.Module):
@@ -86,7 +91,7 @@ class _ModConv(torch.nn.Module):
 
     def modulated_weight(self, w):
         style = self.affine(w)                                       # [1, cin]
-        wt = self.weight * self.scale * style.view(1, -1, 1, 1)
+        wt = self.weight * (style * self.scale).view(1, -1, 1, 1)
         if self.demodulate:
             wt = wt * torch.rsqrt((wt * wt).sum(dim=(1, 2, 3), keepdim=True) + 1e-8)
         return wt
