@@ -54,25 +54,48 @@ class VGGShapedFeatures(torch.nn.Module):
         return feats
 
 
+class _Gram(torch.autograd.Function):
+    """G = F F^T / (c h w) per image of f [n, c, h, w] (a c x c matrix per
+    image), with dL/dF = (dG + dG^T) F / (c h w).  Per-image 2-D products
+    read the features as they lie in memory: no transposed copy of the
+    feature map forward, no strided accumulation of a transposed gradient
+    backward."""
+
+    @staticmethod
+    def forward(ctx, f):
+        n, c, h, w = f.shape
+        fm = f.reshape(n, c, h * w)
+        g = torch.stack([torch.mm(fm[i], fm[i].t()) for i in range(n)]) / (c * h * w)
+        ctx.save_for_backward(f)
+        return g
+
+    @staticmethod
+    def backward(ctx, dg):
+        (f,) = ctx.saved_tensors
+        n, c, h, w = f.shape
+        s = (dg + dg.transpose(1, 2)).contiguous() / (c * h * w)
+        return torch.bmm(s, f.reshape(n, c, h * w)).reshape(n, c, h, w)
+
+
 def gram(f):
-    n, c, h, w = f.shape
-    f = f.reshape(n, c, h * w)
-    return torch.bmm(f, f.transpose(1, 2).contiguous()) / (c * h * w)
+    return _Gram.apply(f)
 
 
 class FeatureLoss:
-    """L2 of Gram matrices on the 5 style layers + pixel L2 (weights w_gram, w_pixel)."""
+    """L2 of Gram matrices on the 5 style layers + pixel L2 (weights w_gram,
+    w_pixel); gram_fn computes the Gram matrices (default: gram)."""
 
-    def __init__(self, net, targets, w_gram=1.0, w_pixel=1.0):
+    def __init__(self, net, targets, w_gram=1.0, w_pixel=1.0, gram_fn=None):
         self.net = net
+        self.gram = gram if gram_fn is None else gram_fn
         self.targets = targets.detach()
         with torch.no_grad():
-            self.target_grams = [gram(f) for f in net(self.targets)]
+            self.target_grams = [self.gram(f) for f in net(self.targets)]
         self.w_gram, self.w_pixel = w_gram, w_pixel
 
     def __call__(self, images):
         feats = self.net(images)
         loss = self.w_pixel * torch.nn.functional.mse_loss(images, self.targets)
         for f, gt in zip(feats, self.target_grams):
-            loss = loss + self.w_gram * torch.nn.functional.mse_loss(gram(f), gt)
+            loss = loss + self.w_gram * torch.nn.functional.mse_loss(self.gram(f), gt)
         return loss

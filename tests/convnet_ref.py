@@ -90,6 +90,14 @@ def synthesis_ref(gen, wp, noises, masks=None, record=None):
     return rgb
 
 
+def gram_ref(f):
+    """Gram matrices per image, F F^T / (c h w), by torch autograd of bmm
+    (the transposed operand made contiguous first)."""
+    n, c, h, w = f.shape
+    f = f.reshape(n, c, h * w)
+    return torch.bmm(f, f.transpose(1, 2).contiguous()) / (c * h * w)
+
+
 def vgg_shaped_ref(net, x, masks=None, record=None):
     """compose.VGGShapedFeatures.forward in x's dtype; masks / record as
     conv3x3_ref, one per convolution."""

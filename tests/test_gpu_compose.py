@@ -45,7 +45,8 @@ def feature_grad(pkg, renderer_cls, tex0, gt, cam, light, P, size, dtype):
         net = _TorchStack(net)
     with torch.no_grad():
         target = r.eval(torch.tensor(gt, device=DEV, dtype=dtype)[None])
-    fl = compose.FeatureLoss(net, target)
+    # the reference side's Gram matrices by torch autograd of bmm
+    fl = compose.FeatureLoss(net, target, gram_fn=None if renderer_cls is None else R.gram_ref)
     t = torch.tensor(tex0, device=DEV, dtype=dtype)[None].requires_grad_(True)
     L = fl(r.eval(t.clamp(-1, 1)))
     L.backward()

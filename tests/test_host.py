@@ -235,3 +235,22 @@ def test_material_recipe_is_uniform_per_channel(pkg):
     np.testing.assert_array_equal(m[3:5], t[3:5])
     with pytest.raises(ValueError):
         syn.ground_truth_textures(8, albedo="pixel")
+
+
+def test_gram_function_against_autograd(pkg):
+    """compose.gram (per-image products, hand-written backward (dG + dG^T) F
+    / (c h w)) against torch autograd of the bmm formula in FP64 on the CPU,
+    and torch.autograd.gradcheck of it."""
+    from svbrdf_amd import compose
+    from tests import convnet_ref as R
+    g = torch.Generator().manual_seed(9)
+    f = torch.randn(3, 5, 4, 6, generator=g, dtype=torch.float64)
+    probe = torch.randn(3, 5, 5, generator=g, dtype=torch.float64)
+    a = f.clone().requires_grad_(True)
+    b = f.clone().requires_grad_(True)
+    ga, gb = compose.gram(a), R.gram_ref(b)
+    torch.testing.assert_close(ga, gb, rtol=1e-12, atol=1e-14)
+    (ga * probe).sum().backward()
+    (gb * probe).sum().backward()
+    torch.testing.assert_close(a.grad, b.grad, rtol=1e-12, atol=1e-14)
+    assert torch.autograd.gradcheck(compose.gram, (f.clone().requires_grad_(True),))
