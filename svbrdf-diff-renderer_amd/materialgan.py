@@ -205,7 +205,7 @@ class VGGLoss(torch.nn.Module):
     def normalize(self, im):
         return (im - self.mean) / self.std
 
-    def compute_feature_vector(self, x):
+    def _weighted_features(self, x):
         feats = []
         for step in self.plan:
             if step == "P":
@@ -214,14 +214,25 @@ class VGGLoss(torch.nn.Module):
             x = self.convs[step](x)
             if step in self.HOOKS:
                 feats.append(x.flatten() * self.weights[len(feats)])
-        return torch.cat(feats)
+        return feats
+
+    def compute_feature_vector(self, x):
+        return torch.cat(self._weighted_features(x))
 
     def load(self, im):
         with torch.no_grad():
             self.im_feature = self.compute_feature_vector(self.normalize(im))
 
     def forward(self, x):
-        return F.mse_loss(self.compute_feature_vector(self.normalize(x)), self.im_feature)
+        # the MSE of the concatenated vectors as a sum over its four segments
+        # (the target's segments are views of the stored vector): the same
+        # loss without copying the features into one vector and back
+        total, off = None, 0
+        for f in self._weighted_features(self.normalize(x)):
+            term = F.mse_loss(f, self.im_feature[off:off + f.numel()], reduction="sum")
+            total = term if total is None else total + term
+            off += f.numel()
+        return total / self.im_feature.numel()
 
 
 class MaterialGANOptim:
