@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SVBRDF_ABI_VERSION 11
+#define SVBRDF_ABI_VERSION 12
 
 enum svbrdf_status {
     SVBRDF_OK = 0,
@@ -372,6 +372,15 @@ enum svbrdf_activation {
  * (cin and cout exchanged) is dL/dx. */
 int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *style, float scale, int demodulate,
                         int transpose, float *packed, void *stream);
+
+/* dL/d(pre-activation) of a sign-keeping activation from dL/dy and the
+ * activation's output y: out = grad_y * (y > 0 ? pos : neg), n elements (ReLU:
+ * pos 1, neg 0; leaky ReLU 0.2 x sqrt(2): sqrt(2), 0.2 sqrt(2)) -- the
+ * backward of the activations the reference's conv blocks apply
+ * (stylegan2_generator_network.py:738-755, descriptor.py:7-79), which it
+ * leaves to torch autograd.  out may alias grad_y. */
+int svbrdf_act_grad(const float *grad_y, const float *y, size_t n, float pos, float neg, float *out,
+                    void *stream);
 
 /* dL/dstyle of the weights svbrdf_conv3x3_pack writes with a style and
  * scale (and demodulation when demodulate != 0), given grad_weight = dL/d(the

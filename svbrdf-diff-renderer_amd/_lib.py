@@ -15,7 +15,7 @@ _PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("SVBRDF_AMD_LIB") or _PKG / "libsvbrdf_hip.so")
 
 # the SVBRDF_ABI_VERSION these signatures describe (include/svbrdf_hip.h)
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 OK = 0
 EINVAL = -1
@@ -82,6 +82,7 @@ SIGNATURES = {
     # convolutions of the MaterialGAN caller's networks
     "svbrdf_conv3x3_pack": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_float, ctypes.c_int,
                                            ctypes.c_int, _vp, _vp]),
+    "svbrdf_act_grad": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     "svbrdf_conv3x3_modulation_grad": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_float,
                                                       ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "svbrdf_conv3x3_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int] * 5),

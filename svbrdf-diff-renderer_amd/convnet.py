@@ -116,13 +116,20 @@ def conv3x3_wgrad(x, grad_z, cout):
     return gw
 
 
+_ACT_SLOPES = {"relu": (1.0, 0.0), "lrelu": (1.4142135623730951, 0.2 * 1.4142135623730951)}
+
+
 def _act_grad(grad_y, y, act):
-    """dL/d(pre-activation) from dL/dy and y: the activations keep the sign."""
-    if act == "relu":
-        return grad_y * (y > 0)
-    if act == "lrelu":
-        return grad_y * torch.where(y > 0, 1.4142135623730951, 0.2 * 1.4142135623730951)
-    return grad_y
+    """dL/d(pre-activation) from dL/dy and y (the activations keep the sign):
+    one HIP kernel, grad_y * (y > 0 ? pos : neg)."""
+    if act not in _ACT_SLOPES:
+        return grad_y
+    g, yy = grad_y.contiguous(), y.contiguous()
+    out = torch.empty_like(g)
+    pos, neg = _ACT_SLOPES[act]
+    _lib.call("svbrdf_act_grad", _lib.ptr(g), _lib.ptr(yy), g.numel(), pos, neg, _lib.ptr(out),
+              _lib.stream_handle(g.device))
+    return out
 
 
 class Conv3x3Function(torch.autograd.Function):

@@ -164,6 +164,25 @@ __global__ __launch_bounds__(kThreads) void conv3x3_pack_kernel(const float *__r
     }
 }
 
+// dL/d(pre-activation) = dL/dy x (y > 0 ? pos : neg): the derivative of a
+// sign-keeping activation read off its output (ReLU: 1 / 0; leaky ReLU x
+// sqrt(2): sqrt(2) / 0.2 sqrt(2)), four elements per thread.
+__global__ __launch_bounds__(kThreads) void act_grad_kernel(const float4 *__restrict__ g, const float4 *__restrict__ y,
+                                                            size_t n4, float pos, float neg, float4 *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n4) return;
+    const float4 gv = g[i], yv = y[i];
+    out[i] = make_float4(gv.x * (yv.x > 0.0f ? pos : neg), gv.y * (yv.y > 0.0f ? pos : neg),
+                         gv.z * (yv.z > 0.0f ? pos : neg), gv.w * (yv.w > 0.0f ? pos : neg));
+}
+
+__global__ __launch_bounds__(kThreads) void act_grad_tail_kernel(const float *__restrict__ g, const float *__restrict__ y,
+                                                                 size_t n, float pos, float neg,
+                                                                 float *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) out[i] = g[i] * (y[i] > 0.0f ? pos : neg);
+}
+
 // Gradient of the modulated, demodulated weights with respect to the style:
 // with u = scale style[ci] w[co][ci][t], d_co = 1/sqrt(sum_{ci,t} u^2 + 1e-8)
 // and the packed weight u d_co, given g = dL/d(u d_co) in [Cout, Cin, 3, 3],
@@ -962,6 +981,25 @@ int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *sty
     else
         hipLaunchKernelGGL(conv3x3_pack_kernel<0>, dim3((unsigned)cout), dim3(kThreads), 0, s, weight, cout, cin, style,
                            scale, dm, tr, packed);
+    return check_launch(what);
+}
+
+int svbrdf_act_grad(const float *grad_y, const float *y, size_t n, float pos, float neg, float *out,
+                    void *stream) {
+    const char *what = "svbrdf_act_grad";
+    if (!grad_y || !y || !out) return fail(SVBRDF_EINVAL, "%s: null pointer argument", what);
+    if (n == 0) return SVBRDF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const bool vec = n % 4 == 0 && ((uintptr_t)grad_y & 15u) == 0 && ((uintptr_t)y & 15u) == 0 &&
+                     ((uintptr_t)out & 15u) == 0;
+    if (vec) {
+        const size_t n4 = n / 4;
+        hipLaunchKernelGGL(act_grad_kernel, dim3((unsigned)((n4 + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                           (const float4 *)grad_y, (const float4 *)y, n4, pos, neg, (float4 *)out);
+    } else {
+        hipLaunchKernelGGL(act_grad_tail_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                           grad_y, y, n, pos, neg, out);
+    }
     return check_launch(what);
 }
 

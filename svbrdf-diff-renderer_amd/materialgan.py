@@ -167,14 +167,17 @@ class StyleGAN2Generator(torch.nn.Module):
         x = self.const
         rgb = None
         wi = 0
+        # one unbind instead of an index per layer: its backward is one stack
+        # of the per-layer gradients, not a zero-filled copy per layer
+        ws = wp.unbind(1)
         for i, conv in enumerate(self.convs):
             # conv + noise_strength * noise + bias, leaky ReLU x sqrt(2), in one kernel
-            x = conv(x, wp[:, min(wi, self.num_layers - 1)], bias=self.bias[i], noise=noises[i],
+            x = conv(x, ws[min(wi, self.num_layers - 1)], bias=self.bias[i], noise=noises[i],
                      noise_strength=self.noise_strength_values[i], act="lrelu")
             wi += 1
             if i == 0 or i % 2 == 0:
                 # toRGB after each resolution's last convolution, skip-summed
-                t = self.to_rgb[(i + 1) // 2](x, wp[:, min(wi, self.num_layers - 1)])
+                t = self.to_rgb[(i + 1) // 2](x, ws[min(wi, self.num_layers - 1)])
                 rgb = t if rgb is None else convnet.upsample2(rgb) + t
         return rgb
 
@@ -284,11 +287,13 @@ class MaterialGANOptim:
 
     def latent_to_textures(self, latent, noises=None):
         t = self.net_obj.synthesis(latent, self.noises if noises is None else noises)
-        textures = t.clone()
-        textures[:, 0:5] = t[:, 0:5].clamp(-1, 1)
-        textures[:, 5] = t[:, 5].clamp(-0.5, 0.5)
-        textures[:, 6:9] = t[:, 6:9].clamp(-1, 1)
-        return textures
+        # channels 0-4 and 6-8 clamped to [-1, 1], roughness (5) to [-0.5, 0.5]:
+        # one clamp with per-channel bounds
+        if getattr(self, "_clamp_bounds", None) is None or self._clamp_bounds[0].device != t.device:
+            lo = torch.full((1, 9, 1, 1), -1.0, device=t.device)
+            lo[:, 5] = -0.5
+            self._clamp_bounds = (lo, -lo)
+        return torch.clamp(t, min=self._clamp_bounds[0], max=self._clamp_bounds[1])
 
     save_loss = staticmethod(SvbrdfOptim.save_loss)
 

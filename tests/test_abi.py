@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib, pkg):
 
 
 def test_queries(lib, pkg):
-    assert lib.svbrdf_abi_version() == 11 == pkg._lib.ABI_VERSION
+    assert lib.svbrdf_abi_version() == 12 == pkg._lib.ABI_VERSION
     # the host-side view of svbrdf_step_state matches the C struct
     assert lib.svbrdf_step_state_bytes() == pkg.optim._STATE_DTYPE.itemsize == 96
     assert lib.svbrdf_max_res() >= 4096
@@ -227,3 +227,9 @@ def test_modulation_grad_validation(lib):
     assert lib.svbrdf_conv3x3_modulation_grad(f, 8, 8, f, 1.0, 1, f, f, f, 63, None) == -3
     assert lib.svbrdf_conv3x3_modulation_grad(f, 8, 8, f, 1.0, 1, f, f, ctypes.c_void_p(0x10004), 64, None) == -1
     assert b"aligned" in lib.svbrdf_last_error()
+
+
+def test_act_grad_validation(lib):
+    f = _fake()
+    assert lib.svbrdf_act_grad(None, f, 16, 1.0, 0.0, f, None) == -1
+    assert lib.svbrdf_act_grad(f, f, 0, 1.0, 0.0, f, None) == 0      # nothing to do

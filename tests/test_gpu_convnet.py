@@ -298,6 +298,27 @@ def test_modulated_conv3x3_against_torch(pkg, demod, cin, cout, side):
         assert e <= 1e-5, (name, e)
 
 
+@pytest.mark.parametrize("act", ["relu", "lrelu"])
+@pytest.mark.parametrize("n", [4096, 4099])
+def test_activation_gradient_kernel_is_torch_bitwise(pkg, act, n):
+    """svbrdf_act_grad (four elements per thread, and the one-element tail
+    path for odd counts) against torch's dL/dy * act'(y) from y, bit for bit,
+    with NaN, infinite and exactly-zero y and dL/dy among the values."""
+    from svbrdf_amd import convnet
+    g = torch.Generator().manual_seed(n)
+    gy = torch.randn(n, generator=g)
+    y = torch.randn(n, generator=g)
+    y[:3] = torch.tensor([0.0, float("nan"), float("inf")])
+    gy[3:6] = torch.tensor([float("inf"), -0.0, float("nan")])
+    gy, y = gy.to(DEV), y.to(DEV)
+    ref = gy * (y > 0) if act == "relu" else gy * torch.where(y > 0, 1.4142135623730951, 0.2 * 1.4142135623730951)
+    out = convnet._act_grad(gy, y, act)
+    torch.cuda.synchronize()
+    assert torch.equal(torch.isnan(out), torch.isnan(ref))
+    ok = ~torch.isnan(ref)
+    assert torch.equal(out[ok].view(torch.int32), ref[ok].view(torch.int32))
+
+
 def test_generator_synthesis_and_gradients(pkg):
     """The 256^2 StyleGAN2-shaped synthesis on the HIP convolutions against its
     torch restatement in FP32 and FP64: the 9-channel output, and the
