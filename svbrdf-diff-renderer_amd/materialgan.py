@@ -208,7 +208,7 @@ class VGGLoss(torch.nn.Module):
     def normalize(self, im):
         return (im - self.mean) / self.std
 
-    def _weighted_features(self, x):
+    def _features(self, x):
         feats = []
         for step in self.plan:
             if step == "P":
@@ -216,25 +216,28 @@ class VGGLoss(torch.nn.Module):
                 continue
             x = self.convs[step](x)
             if step in self.HOOKS:
-                feats.append(x.flatten() * self.weights[len(feats)])
+                feats.append(x.flatten())
         return feats
 
     def compute_feature_vector(self, x):
-        return torch.cat(self._weighted_features(x))
+        return torch.cat([f * w for f, w in zip(self._features(x), self.weights)])
 
     def load(self, im):
         with torch.no_grad():
-            self.im_feature = self.compute_feature_vector(self.normalize(im))
+            feats = self._features(self.normalize(im))
+            self.im_feature = torch.cat([f * w for f, w in zip(feats, self.weights)])
+            self._targets = feats
 
     def forward(self, x):
-        # the MSE of the concatenated vectors as a sum over its four segments
-        # (the target's segments are views of the stored vector): the same
-        # loss without copying the features into one vector and back
-        total, off = None, 0
-        for f in self._weighted_features(self.normalize(x)):
-            term = F.mse_loss(f, self.im_feature[off:off + f.numel()], reduction="sum")
+        # mean((w_l f_l - w_l t_l)^2) over the concatenated vector, taken per
+        # segment as w_l^2 sum((f_l - t_l)^2): no weighted copies of the
+        # feature maps and no concatenation, forward or backward
+        total = None
+        for f, t, w in zip(self._features(self.normalize(x)), self._targets, self.weights):
+            if w == 0.0:
+                continue
+            term = F.mse_loss(f, t, reduction="sum") * (w * w)
             total = term if total is None else total + term
-            off += f.numel()
         return total / self.im_feature.numel()
 
 
