@@ -98,9 +98,12 @@ class _ModConv(torch.nn.Module):
 
     def forward(self, x, w, bias=None, noise=None, noise_strength=0.0, act="none"):
         if self.k == 1:
-            # a 1x1 convolution is a plain GEMM: [Cout, Cin] x [N, Cin, H W]
+            # a 1x1 convolution is a plain GEMM: [Cout, Cin] x [Cin, H W] per
+            # sample (torch.mm for the one sample: no broadcast copies)
             wt = self.modulated_weight(w)
             n, c, h, wd = x.shape
+            if n == 1:
+                return torch.mm(wt.view(wt.shape[0], c), x.reshape(c, h * wd)).view(1, -1, h, wd)
             return torch.matmul(wt.view(wt.shape[0], c), x.reshape(n, c, h * wd)).view(n, -1, h, wd)
         # 3x3: the modulation and demodulation happen inside the HIP weight
         # packing, and the style's gradient comes from the HIP modulation
