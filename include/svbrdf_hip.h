@@ -377,7 +377,8 @@ int svbrdf_conv3x3_pack(const float *weight, int cout, int cin, const float *sty
  * activation's output y: out = grad_y * (y > 0 ? pos : neg), n elements (ReLU:
  * pos 1, neg 0; leaky ReLU 0.2 x sqrt(2): sqrt(2), 0.2 sqrt(2)) -- the
  * backward of the activations the reference's conv blocks apply
- * (stylegan2_generator_network.py:738-755, descriptor.py:7-79), which it
+ * (src/higan_models/stylegan2_generator_network.py:738-755,
+ * src/descriptor.py:7-79), which it
  * leaves to torch autograd.  out may alias grad_y. */
 int svbrdf_act_grad(const float *grad_y, const float *y, size_t n, float pos, float neg, float *out,
                     void *stream);
@@ -386,8 +387,9 @@ int svbrdf_act_grad(const float *grad_y, const float *y, size_t n, float pos, fl
  * scale (and demodulation when demodulate != 0), given grad_weight = dL/d(the
  * packed weight) in the natural [cout, cin, 3, 3] layout (what
  * svbrdf_conv3x3_wgrad returns): the backward of the modulation and
- * demodulation of ModulateConvBlock.forward, stylegan2_generator_network.py:
- * 694-706, which the reference leaves to torch autograd.  grad_style [cin] is
+ * demodulation of ModulateConvBlock.forward,
+ * src/higan_models/stylegan2_generator_network.py:694-706, which the
+ * reference leaves to torch autograd.  grad_style [cin] is
  * overwritten.  workspace: 8-byte aligned, at least 2 cout floats. */
 int svbrdf_conv3x3_modulation_grad(const float *weight, int cout, int cin, const float *style, float scale,
                                    int demodulate, const float *grad_weight, float *grad_style, void *workspace,
