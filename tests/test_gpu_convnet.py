@@ -260,12 +260,15 @@ def _gradient_check(g, g64m, g64, g32, label):
 
 @pytest.mark.parametrize("demod,cin,cout,side", [(True, 64, 32, 16), (False, 32, 64, 8), (True, 512, 512, 4),
                                                 (True, 128, 256, 32)])
-def test_modulated_conv3x3_against_torch(pkg, demod, cin, cout, side):
+@pytest.mark.parametrize("act,style_grad", [("none", True), ("lrelu", True), ("lrelu", False)])
+def test_modulated_conv3x3_against_torch(pkg, demod, cin, cout, side, act, style_grad):
     """modulated_conv3x3 -- the style modulation and demodulation inside the
     HIP weight packing, dL/dstyle from svbrdf_conv3x3_modulation_grad --
     against the torch restatement of ModulateConvBlock (weights scaled by the
     style, demodulated, then F.conv2d + noise + bias) in FP64: the output,
-    dL/dx, dL/dstyle and dL/dnoise (no activation, so no branch kinks)."""
+    dL/dx, dL/dstyle and dL/dnoise, also without a style gradient (the noise
+    epochs).  With the leaky ReLU the FP64 side takes its branches from the
+    GPU output's signs."""
     from svbrdf_amd import convnet
     g = torch.Generator().manual_seed(cin + cout + side)
     x = torch.randn(1, cin, side, side, generator=g)
@@ -276,23 +279,29 @@ def test_modulated_conv3x3_against_torch(pkg, demod, cin, cout, side):
     probe = torch.randn(1, cout, side, side, generator=g)
     scale = 1.0 / math.sqrt(cin * 9)
 
-    def ref(xx, ss, nn):
+    def ref(xx, ss, nn, mask):
         wt = w.double() * scale * ss.view(1, -1, 1, 1)
         if demod:
             wt = wt * torch.rsqrt((wt * wt).sum(dim=(1, 2, 3), keepdim=True) + 1e-8)
-        return R.conv3x3_ref(xx, wt, bias.double(), nn, 0.3, "none")
+        return R.conv3x3_ref(xx, wt, bias.double(), nn, 0.3, act, mask)
 
-    xs, ss, ns = (t.to(DEV).requires_grad_(True) for t in (x, style, noise))
-    y = convnet.modulated_conv3x3(xs, ss, w.to(DEV), scale, demod, bias.to(DEV), ns, 0.3, "none")
+    xs, ns = x.to(DEV).requires_grad_(True), noise.to(DEV).requires_grad_(True)
+    ss = style.to(DEV).requires_grad_(style_grad)
+    y = convnet.modulated_conv3x3(xs, ss, w.to(DEV), scale, demod, bias.to(DEV), ns, 0.3, act)
     (y * probe.to(DEV)).sum().backward()
-    x64, s64, n64 = (t.double().requires_grad_(True) for t in (x, style, noise))
-    y64 = ref(x64, s64, n64)
-    (y64 * probe.double()).sum().backward()
     torch.cuda.synchronize()
+    mask = (y.detach() > 0).cpu() if act == "lrelu" else None
+    x64, s64, n64 = (t.double().requires_grad_(True) for t in (x, style, noise))
+    y64 = ref(x64, s64, n64, mask)
+    (y64 * probe.double()).sum().backward()
     ey = float((y.detach().cpu().double() - y64.detach()).abs().max() / y64.abs().max())
     assert ey <= 1e-5, ey
-    for got, want, name in ((xs.grad, x64.grad, "dL/dx"), (ss.grad, s64.grad, "dL/dstyle"),
-                            (ns.grad, n64.grad, "dL/dnoise")):
+    checks = [(xs.grad, x64.grad, "dL/dx"), (ns.grad, n64.grad, "dL/dnoise")]
+    if style_grad:
+        checks.append((ss.grad, s64.grad, "dL/dstyle"))
+    else:
+        assert ss.grad is None
+    for got, want, name in checks:
         e = float((got.cpu().double() - want).norm() / want.norm())
         print(f"{name}: relative error {e:.2e}")
         assert e <= 1e-5, (name, e)
